@@ -1,0 +1,146 @@
+"""Benchmark: simulated milliseconds per wall second of the coupled epoch loop.
+
+Workload: the coupled layered-stack run at 256x256x192 voxels of 15 um (seeded
+7-layer stack under 21 voxels of ambient air, 40 W at 22 deg, 0.5 mm spot,
+1 ms epochs of 150k photons + 20 sub-steps of 50 us, boil 100 degC, ablate
+350 degC, Arrhenius char, diffusion plume on a 128^3 grid), synthetic inputs,
+state resident in HBM before the timed region.  One step = one epoch = 1 ms
+simulated.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bug-defense_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per voxel of one fused sub-step with char on, Arrhenius off:
+# read T, k, rho_cp, Q, write T (20 B) + H r/w (8) + chi r/w (8) + dry (1) + ablated (1)
+SUBSTEP_BYTES_PER_VOXEL = 38
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(kw, epochs, threads):
+    """The FP64 oracle engine on the host cores, same workload, a bounded number of epochs."""
+    from oracle import oracle as O
+    okw = dict(kw)
+    okw.pop('plume_grid', None)
+    eng = O.OracleEngine(mc_threads=threads, plume_grid=kw.get('plume_grid'), **okw)
+    eng.step_epoch()  # first-touch of the arrays outside the timed span
+    t0 = time.perf_counter()
+    for _ in range(epochs):
+        eng.step_epoch()
+    dt = time.perf_counter() - t0
+    return epochs * eng.dt_epoch * 1e3 / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-epochs", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
+                                   "(the multi-GPU sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    from simulation import _bdlib as B
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.workloads import layered_stack
+
+    if B.device_count() < 1:
+        raise RuntimeError("bench: no HIP device visible")
+    kw = layered_stack()
+    t0 = time.perf_counter()
+    eng = CoupledAblationEngine(**kw)
+    log(f"bench: engine built in {time.perf_counter() - t0:.1f}s ({eng.Nx}x{eng.Ny}x{eng.Nz})")
+    for i in range(args.warmup):
+        eng.step_epoch()
+    log(f"bench: {args.warmup} warmup epochs done")
+    eng._ctx.sync()
+    sub_ms, mc_ms, opt_ms, th_ms, ph_ms, events, nnew = [], [], [], [], [], 0.0, 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        d = eng.step_epoch()
+        ld = eng.last_diag
+        sub_ms.append(ld.t_substeps_ms)
+        mc_ms.append(ld.t_mc_kernel_ms)
+        opt_ms.append(ld.t_optical_ms)
+        th_ms.append(ld.t_thermal_ms)
+        ph_ms.append(ld.t_phase_ms)
+        events += ld.mc_events
+        nnew += d['n_new']
+        if (i + 1) % 25 == 0:
+            log(f"bench: epoch {i + 1}/{args.steps}")
+    eng._ctx.sync()
+    wall = time.perf_counter() - t0
+    sim_ms_per_s = args.steps * eng.dt_epoch * 1e3 / wall
+    nvox = eng.Nx * eng.Ny * eng.Nz
+    n_sub = eng.n_substeps
+    sub_launch_ms = float(np.mean(sub_ms)) / n_sub
+    achieved = SUBSTEP_BYTES_PER_VOXEL * nvox / (sub_launch_ms * 1e-3) / 1e9
+    mc_avg = float(np.mean(mc_ms))
+    result = {
+        "metric": "simulated ms/s at 256x256x192 (plain MC path); photons/s",
+        "value": round(sim_ms_per_s, 3),
+        "unit": "simulated ms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (transport, stencils); f64 ledgers",
+        "data": "synthetic (seeded layered stack, no datasets)",
+        "config": {
+            "workload": "coupled layered stack, plain MC path: 256x256x192 voxels of 15 um, 7 layers + 21 voxels "
+                        "ambient air, 40 W at 22 deg, 0.5 mm spot, 1 ms epochs of 150k photons + 20 x 50 us "
+                        "sub-steps, boil 100 C, ablate 350 C, Arrhenius char, diffusion plume on 128^3",
+            "voxels": nvox, "photons_per_epoch": eng.n_photons, "substeps_per_epoch": n_sub,
+        },
+        "photons_per_s": round(eng.n_photons / (mc_avg * 1e-3), 1),
+        "mc_events_per_s": round(events / (sum(mc_ms) * 1e-3), 1),
+        "breakdown_ms": {"optical": round(float(np.mean(opt_ms)), 4), "thermal": round(float(np.mean(th_ms)), 4),
+                         "substeps": round(float(np.mean(sub_ms)), 4), "mc_kernel": round(mc_avg, 4),
+                         "phase_plume": round(float(np.mean(ph_ms)), 4)},
+        "removed_voxels": int(nnew),
+        "roofline": {
+            "kernel": "k_substep (fused bioheat sub-step)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_voxel": SUBSTEP_BYTES_PER_VOXEL,
+            "avg_launch_ms": round(sub_launch_ms, 5),
+        },
+    }
+    eng.close()
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"bench: cpu baseline ({args.cpu_epochs} epochs, {threads} threads)")
+        v, secs = cpu_baseline(kw, args.cpu_epochs, threads)
+        result["cpu_baseline"] = {"value": round(v, 4), "unit": "simulated ms/s", "cores": threads, "kind": "port",
+                                  "sample": f"{args.cpu_epochs} epochs of the same workload on the FP64 oracle "
+                                            f"engine ({secs:.1f} s)"}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

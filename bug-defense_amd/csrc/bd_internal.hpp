@@ -1,0 +1,139 @@
+// bd_internal.hpp -- shared definitions of the gfx950 implementation behind bugdef.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/bugdef.h"
+
+namespace bd {
+
+// Flags byte per voxel: ablated / dry / "banked enthalpy > 0".  Keeping the
+// three state bits in one byte lets the sub-step read one byte instead of two
+// masks and skip the H_acc read for every cell that has nothing banked.
+constexpr uint8_t F_ABL = 1;
+constexpr uint8_t F_DRY = 2;
+constexpr uint8_t F_HPOS = 4;
+
+// Device buffers of a context.  Unused buffers stay null until first use.
+struct Field {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int64_t n = 0;   // element count (1 for a broadcast scalar parameter)
+};
+
+struct Ctx {
+    bd_grid g{};
+    int64_t nvox = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
+    bd_physics phys{};
+    bool has_phys = false;
+    bd_plume_params plume{};
+    int64_t pvox = 0;
+
+    // state (f32 unless noted)
+    float *T = nullptr, *T2 = nullptr;   // ping-pong
+    float *H = nullptr, *Om = nullptr, *chi = nullptr, *Q = nullptr;
+    float *k = nullptr, *rc = nullptr, *hlat = nullptr;
+    uint8_t *flags = nullptr;            // F_ABL | F_DRY | F_HPOS
+    float *mu_a = nullptr, *mu_s = nullptr, *gg = nullptr, *nn = nullptr;
+    float *wet_mu_a = nullptr, *wet_k = nullptr, *wet_rc = nullptr;
+    float *Tpeak = nullptr;
+    double *abl_time = nullptr;
+    float2 *AL = nullptr;                // interleaved absorbed / path tallies
+    float4 *optics = nullptr;            // packed transport optics {mu_a, mu_s, g, n}
+    float *P = nullptr, *P2 = nullptr, *Pext = nullptr;  // plume grid
+    uint8_t *pvoid = nullptr;
+    uint8_t *to_remove = nullptr;
+    Field ds[7];                         // dry state: mu_a mu_s g n k rho_cp t_remove
+    Field cs[5];                         // char state: mu_a mu_s k rho_cp t_remove
+    float *kb[3] = {nullptr, nullptr, nullptr};  // blast splat taps
+    float *kl[3] = {nullptr, nullptr, nullptr};  // legacy splat taps
+    int nkb[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+    int nkl[3] = {0, 0, 0}, rl[3] = {0, 0, 0};
+    float *splat_tmp1 = nullptr, *splat_tmp2 = nullptr;
+    size_t splat_tmp_bytes = 0;
+
+    // scratch
+    double *red = nullptr;               // reduction accumulators (64 doubles)
+    double *lat = nullptr;               // lateral escape per depth
+    int64_t lat_n = 0;
+    unsigned long long *counter = nullptr;  // photon work counter + misc ints
+    void *staging = nullptr;             // f64 staging for conversions
+    size_t staging_bytes = 0;
+    void *host_pinned = nullptr;         // small pinned host block (64 doubles)
+
+    // tuned launch shape of the sub-step kernel
+    int tile_by = 4, tile_xc = 16;
+    bool tuned = false;
+    double last_ledger_conducted = 0.0;
+};
+
+void set_error(const std::string &msg);
+
+#define BD_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t _e = (call);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            ::bd::set_error(std::string(#call) + ": " + hipGetErrorString(_e));          \
+            return BD_EHIP;                                                              \
+        }                                                                                \
+    } while (0)
+
+#define BD_CHECK(cond, msg)                                                              \
+    do {                                                                                 \
+        if (!(cond)) {                                                                   \
+            ::bd::set_error(msg);                                                        \
+            return BD_EINVAL;                                                            \
+        }                                                                                \
+    } while (0)
+
+#define BD_TRY(expr)                                                                     \
+    do {                                                                                 \
+        int _s = (expr);                                                                 \
+        if (_s != BD_OK) return _s;                                                      \
+    } while (0)
+
+int ensure(Ctx *c, void **p, size_t bytes);  // lazily allocate (and zero) a buffer
+int ensure_state(Ctx *c);                     // T, T2, H, chi, Q, k, rc, flags, ...
+int ensure_staging(Ctx *c, size_t bytes);
+
+// launch helpers implemented in the kernel translation units
+int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger);
+int launch_plume_substeps(Ctx *c, int n, double dt, double decay);
+int launch_plateau(Ctx *c);
+int launch_arrhenius(Ctx *c, double A, double EaR, double dt);
+int launch_char(Ctx *c, double A, double EaR, double dt);
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out);
+int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J);
+int launch_plume_void(Ctx *c);
+int launch_plume_inject(Ctx *c);
+int launch_plume_sync(Ctx *c);
+int launch_plume_update_sep(Ctx *c, double dt_epoch);
+int launch_plume_diag(Ctx *c, double out[4]);
+int launch_plume_splat_box(Ctx *c, int which, const float *host_box, int Lx, int Ly, int Lz, int x0, int y0, int z0,
+                           double delta);
+int launch_pack_optics(Ctx *c, bool add_plume);
+int launch_reduce(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t mask_bits,
+                  double *out);
+int launch_fill(Ctx *c, void *dst, int kind, int64_t n, double v);
+int launch_phase_map(Ctx *c, const uint8_t *air_gap_dev, uint8_t *out_dev);
+int launch_slice(Ctx *c, const void *src, int kind, int plane, int index, int stride, float *out, int64_t n);
+
+// element kinds of device buffers
+constexpr int K_F32 = 0, K_F64 = 1, K_U8 = 2, K_FLAG_ABL = 3, K_FLAG_DRY = 4, K_AL_A = 5, K_AL_L = 6;
+
+inline int blocks_for(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    return (int)(b < 1 ? 1 : (b > 1048576 ? 1048576 : b));
+}
+
+}  // namespace bd
+
+struct bd_ctx {
+    bd::Ctx c;
+};

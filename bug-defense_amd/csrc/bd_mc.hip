@@ -1,0 +1,607 @@
+// bd_mc.hip -- voxel Monte Carlo photon transport for gfx950.
+//
+// Physics: voxel_mc_3d.py:33-337 (_fresnel_unpolarized, _refract_axis,
+// _dist_to_voxel_exit, _sample_beam_offsets, _trace_one_photon) -- optical-depth
+// marching voxel by voxel, Henyey-Greenstein scattering, Russian roulette,
+// Fresnel/TIR and Snell refraction at every index step and at the entrance and
+// exit faces; the CPU semantics (cos_th_safe = max(cos, 0.05), |dn| > 1e-9,
+// r_max after every move, lateral escape per depth) are kept.
+//
+// Execution: a persistent kernel with one photon packet per lane.  When a
+// lane's packet terminates, the wave refills every idle lane with one atomic
+// on a global photon counter (ballot + prefix rank), so the wave stays full
+// while photons of very different lengths finish.  Each photon id owns its
+// own Philox4x32-10 stream (key = seed, counter = (draw block, id, tag)), so the
+// traced photons do not depend on scheduling; only the float-atomic summation
+// order of the voxel tallies does.
+//
+// Tallies: absorbed weight A and path length L are interleaved per voxel
+// (float2) so the two atomics of a collision hit one cache line, and each lane
+// coalesces consecutive contributions to the same voxel in registers before
+// issuing them (one atomic pair per voxel visit, not per event).  The scalar
+// tallies (specular, diffuse, transmitted, lateral, absorbed, events) are kept
+// per lane in FP64 and reduced once per wave.
+#include "bd_internal.hpp"
+
+#include <chrono>
+#include <cmath>
+
+namespace bd {
+
+struct McArgs {
+    const float4 *__restrict__ opt;   // {mu_a, mu_s, g, n}
+    float2 *__restrict__ AL;
+    double *__restrict__ acc;         // 0 spec, 1 diffuse, 2 trans, 3 lateral, 4 absorbed, 5 events
+    double *__restrict__ lat;
+    unsigned long long *__restrict__ counter;
+    int Nx, Ny, Nz;
+    float dx, dy, dz, Lx, Ly, Lz;
+    float sigma, R;
+    int profile, la;
+    float iu0, iu1, iu2;
+    double cos_in_d;                  // entrance direction cosine, FP64
+    float cos_ph, sin_ph, cts;
+    float xc, yc, zc;
+    double n_amb;
+    float rmax2;
+    long long n_photons, photon_offset, max_events;
+    unsigned k0, k1;
+    int score_L;
+};
+
+constexpr unsigned TAG = 0x6d63u;
+
+struct Rng {
+    unsigned k0, k1, p0, p1, blk;
+    unsigned b0, b1, b2, b3;
+    int idx;
+};
+
+__device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1,
+                                       unsigned &o0, unsigned &o1, unsigned &o2, unsigned &o3)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const unsigned n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    o0 = c0;
+    o1 = c1;
+    o2 = c2;
+    o3 = c3;
+}
+
+__device__ __forceinline__ float rng_u(Rng &r)
+{
+    if (r.idx == 4) {
+        philox(r.blk, r.p0, r.p1, TAG, r.k0, r.k1, r.b0, r.b1, r.b2, r.b3);
+        r.blk++;
+        r.idx = 0;
+    }
+    const unsigned x = r.idx == 0 ? r.b0 : (r.idx == 1 ? r.b1 : (r.idx == 2 ? r.b2 : r.b3));
+    r.idx++;
+    return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ float fresnel(float n1, float n2, float ci)
+{
+    if (fabsf(n1 - n2) < 1e-9f) return 0.f;
+    ci = fminf(1.f, fabsf(ci));
+    if (ci > 0.999999f) {
+        const float r = (n1 - n2) / (n1 + n2);
+        return r * r;
+    }
+    const float e = n1 / n2;
+    const float st2 = e * e * (1.f - ci * ci);
+    if (st2 >= 1.f) return 1.f;
+    const float ct = sqrtf(1.f - st2);
+    const float rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+    const float rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+__device__ __forceinline__ double fresnel_d(double n1, double n2, double ci)
+{
+    if (fabs(n1 - n2) < 1e-9) return 0.0;
+    ci = fmin(1.0, fabs(ci));
+    if (ci > 0.999999) {
+        const double r = (n1 - n2) / (n1 + n2);
+        return r * r;
+    }
+    const double st2 = (n1 / n2) * (n1 / n2) * (1.0 - ci * ci);
+    if (st2 >= 1.0) return 1.0;
+    const double ct = sqrt(1.0 - st2);
+    const double rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+    const double rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+    return 0.5 * (rs * rs + rp * rp);
+}
+
+__device__ __forceinline__ void refract(float &ua, float &ub, float &uc, float eta)
+{
+    // ua: component normal to the face; ub, uc tangential
+    const float sgn = ua > 0.f ? 1.f : -1.f;
+    ub *= eta;
+    uc *= eta;
+    ua = sgn * sqrtf(fmaxf(0.f, 1.f - ub * ub - uc * uc));
+}
+
+struct Photon {
+    float x, y, z, ux, uy, uz, w, tau;
+    int ix, iy, iz;
+    float4 o;
+    long long ev;
+};
+
+__global__ __launch_bounds__(256) void k_mc(McArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
+    const int depth_axis = a.la == 0 ? 2 : 0;
+    Photon p;
+    Rng rng;
+    bool alive = false;
+    bool exhausted = false;
+    int pv = -1;
+    float pA = 0.f, pL = 0.f;
+    double t_spec = 0.0, t_diff = 0.0, t_trans = 0.0, t_lat = 0.0, t_abs = 0.0;
+    long long t_ev = 0;
+
+    auto flush = [&]() {
+        if (pv >= 0) {
+            float *cell = (float *)&a.AL[pv];
+            if (pA != 0.f) unsafeAtomicAdd(cell, pA);
+            if (a.score_L && pL != 0.f) unsafeAtomicAdd(cell + 1, pL);
+        }
+        pv = -1;
+        pA = 0.f;
+        pL = 0.f;
+    };
+    auto tally = [&](int v, float dA, float dL) {
+        if (v != pv) {
+            flush();
+            pv = v;
+        }
+        pA += dA;
+        pL += dL;
+    };
+    auto die_lateral = [&](int depth_idx, float wt) {
+        t_lat += (double)wt;
+        unsafeAtomicAdd(&a.lat[depth_idx], (double)wt);
+        alive = false;
+        flush();
+    };
+
+    for (;;) {
+        // ---- refill idle lanes from the global photon counter
+        const bool need = !alive && !exhausted;
+        const unsigned long long mask = __ballot(need);
+        if (mask) {
+            const int cnt = __popcll(mask);
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)cnt);
+            base = __shfl(base, leader);
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            if (need) {
+                const unsigned long long my = base + (unsigned long long)rank;
+                if (my >= (unsigned long long)a.n_photons) {
+                    exhausted = true;
+                } else {
+                    // ---- launch (voxel_mc_3d.py:150-194)
+                    const unsigned long long pid = (unsigned long long)a.photon_offset + my;
+                    rng.k0 = a.k0;
+                    rng.k1 = a.k1;
+                    rng.p0 = (unsigned)pid;
+                    rng.p1 = (unsigned)(pid >> 32);
+                    rng.blk = 0;
+                    rng.idx = 4;
+                    float u1, u2;
+                    if (a.profile == 1) {
+                        const float rr = a.R * sqrtf(rng_u(rng));
+                        float s, c;
+                        __sincosf(6.283185307179586f * rng_u(rng), &s, &c);
+                        u1 = rr * c;
+                        u2 = rr * s;
+                    } else if (a.profile == 2 || a.profile == 3) {
+                        for (;;) {
+                            const float rn = sqrtf(rng_u(rng));
+                            const float accp = a.profile == 2 ? __expf(-2.f * rn) : __expf(-2.f * rn * rn);
+                            if (rng_u(rng) < accp) {
+                                float s, c;
+                                __sincosf(6.283185307179586f * rng_u(rng), &s, &c);
+                                u1 = a.R * rn * c;
+                                u2 = a.R * rn * s;
+                                break;
+                            }
+                        }
+                    } else {
+                        const float ua = rng_u(rng), ub = rng_u(rng);
+                        const float rad = sqrtf(-2.f * __logf(ua));
+                        float s, c;
+                        __sincosf(6.283185307179586f * ub, &s, &c);
+                        u1 = a.sigma * rad * c;
+                        u2 = a.sigma * rad * s;
+                    }
+                    bool out;
+                    if (a.la == 0) {
+                        p.x = a.xc + (u1 / a.cts) * a.cos_ph - u2 * a.sin_ph;
+                        p.y = a.yc + (u1 / a.cts) * a.sin_ph + u2 * a.cos_ph;
+                        p.z = 0.f;
+                        const float ddx = p.x - a.xc, ddy = p.y - a.yc;
+                        out = p.x < 0.f || p.x >= a.Lx || p.y < 0.f || p.y >= a.Ly ||
+                              (a.rmax2 > 0.f && ddx * ddx + ddy * ddy >= a.rmax2);
+                        p.ix = min((int)(p.x / a.dx), Nx - 1);
+                        p.iy = min((int)(p.y / a.dy), Ny - 1);
+                        p.iz = 0;
+                    } else {
+                        p.x = 0.f;
+                        p.y = a.yc + (u1 / a.cts) * a.cos_ph - u2 * a.sin_ph;
+                        p.z = a.zc + (u1 / a.cts) * a.sin_ph + u2 * a.cos_ph;
+                        const float ddy = p.y - a.yc, ddz = p.z - a.zc;
+                        out = p.y < 0.f || p.y >= a.Ly || p.z < 0.f || p.z >= a.Lz ||
+                              (a.rmax2 > 0.f && ddy * ddy + ddz * ddz >= a.rmax2);
+                        p.ix = 0;
+                        p.iy = min((int)(p.y / a.dy), Ny - 1);
+                        p.iz = min((int)(p.z / a.dz), Nz - 1);
+                    }
+                    if (out) {
+                        t_lat += 1.0;
+                        unsafeAtomicAdd(&a.lat[0], 1.0);
+                    } else {
+                        p.ux = a.iu0;
+                        p.uy = a.iu1;
+                        p.uz = a.iu2;
+                        p.o = a.opt[((int64_t)p.ix * Ny + p.iy) * Nz + p.iz];
+                        const double r_entry = fresnel_d(a.n_amb, (double)p.o.w, a.cos_in_d);
+                        t_spec += r_entry;
+                        p.w = (float)(1.0 - r_entry);
+                        if (p.w > 0.f) {
+                            if (fabs((double)p.o.w - a.n_amb) > 1e-9) {
+                                const float eta = (float)(a.n_amb / (double)p.o.w);
+                                if (depth_axis == 2) refract(p.uz, p.ux, p.uy, eta);
+                                else refract(p.ux, p.uy, p.uz, eta);
+                            }
+                            p.tau = -__logf(fmaxf(rng_u(rng), 1e-12f));
+                            p.ev = 0;
+                            alive = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (!__any(alive)) {
+            if (__all(exhausted)) break;
+            continue;
+        }
+        // ---- transport a burst of events, then refill
+        for (int it = 0; it < 16; ++it) {
+            if (!alive) break;
+            const int v = (p.ix * Ny + p.iy) * Nz + p.iz;
+            const float mua = p.o.x;
+            const float mut = p.o.x + p.o.y;
+            // _dist_to_voxel_exit (voxel_mc_3d.py:72-98)
+            const float sx = p.ux > 1e-15f ? ((p.ix + 1) * a.dx - p.x) / p.ux
+                           : (p.ux < -1e-15f ? (p.ix * a.dx - p.x) / p.ux : 1e30f);
+            const float sy = p.uy > 1e-15f ? ((p.iy + 1) * a.dy - p.y) / p.uy
+                           : (p.uy < -1e-15f ? (p.iy * a.dy - p.y) / p.uy : 1e30f);
+            const float sz = p.uz > 1e-15f ? ((p.iz + 1) * a.dz - p.z) / p.uz
+                           : (p.uz < -1e-15f ? (p.iz * a.dz - p.z) / p.uz : 1e30f);
+            int axis;
+            float s_exit;
+            if (sx <= sy && sx <= sz) { axis = 0; s_exit = sx; }
+            else if (sy <= sz) { axis = 1; s_exit = sy; }
+            else { axis = 2; s_exit = sz; }
+            s_exit = fmaxf(s_exit, 0.f);
+            t_ev += 1;
+            p.ev++;
+            if (mut > 1e-12f && mut * s_exit >= p.tau) {
+                // ---- interaction inside the voxel (voxel_mc_3d.py:201-265)
+                const float s_col = p.tau / mut;
+                tally(v, 0.f, p.w * s_col);
+                p.x += s_col * p.ux;
+                p.y += s_col * p.uy;
+                p.z += s_col * p.uz;
+                if (a.rmax2 > 0.f) {
+                    const float rr = a.la == 0 ? (p.x - a.xc) * (p.x - a.xc) + (p.y - a.yc) * (p.y - a.yc)
+                                               : (p.y - a.yc) * (p.y - a.yc) + (p.z - a.zc) * (p.z - a.zc);
+                    if (rr >= a.rmax2) {
+                        die_lateral(a.la == 0 ? p.iz : p.ix, p.w);
+                        continue;
+                    }
+                }
+                const float dw = p.w * (mua / mut);
+                p.w -= dw;
+                tally(v, dw, 0.f);
+                t_abs += (double)dw;
+                if (p.w < 1e-4f) {
+                    if (rng_u(rng) < 0.1f) p.w *= 10.f;
+                    else {
+                        alive = false;
+                        flush();
+                        continue;
+                    }
+                }
+                const float gl = p.o.z;
+                const float xi = rng_u(rng);
+                float ct;
+                if (fabsf(gl) < 1e-4f) ct = 2.f * xi - 1.f;
+                else {
+                    const float tmp = (1.f - gl * gl) / (1.f - gl + 2.f * gl * xi);
+                    ct = (1.f + gl * gl - tmp * tmp) / (2.f * gl);
+                    ct = fmaxf(-1.f, fminf(1.f, ct));
+                }
+                const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+                float sp, cp;
+                __sincosf(6.283185307179586f * rng_u(rng), &sp, &cp);
+                if (fabsf(p.uz) > 0.99999f) {
+                    const float sg = p.uz > 0.f ? 1.f : -1.f;
+                    p.ux = st * cp;
+                    p.uy = st * sp;
+                    p.uz = ct * sg;
+                } else {
+                    const float den = sqrtf(fmaxf(1.f - p.uz * p.uz, 1e-12f));
+                    const float nx = st * (p.ux * p.uz * cp - p.uy * sp) / den + p.ux * ct;
+                    const float ny = st * (p.uy * p.uz * cp + p.ux * sp) / den + p.uy * ct;
+                    const float nz = -st * cp * den + p.uz * ct;
+                    p.ux = nx;
+                    p.uy = ny;
+                    p.uz = nz;
+                }
+                const float nrm = sqrtf(p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
+                if (nrm > 1e-12f) {
+                    p.ux /= nrm;
+                    p.uy /= nrm;
+                    p.uz /= nrm;
+                }
+                p.tau = -__logf(fmaxf(rng_u(rng), 1e-12f));
+            } else {
+                // ---- full chord to the exit face (voxel_mc_3d.py:267-337)
+                p.tau -= mut * s_exit;
+                tally(v, 0.f, p.w * s_exit);
+                p.x += s_exit * p.ux;
+                p.y += s_exit * p.uy;
+                p.z += s_exit * p.uz;
+                int jx = p.ix, jy = p.iy, jz = p.iz;
+                int sdir;
+                if (axis == 0) {
+                    sdir = p.ux > 0.f ? 1 : -1;
+                    jx += sdir;
+                    p.x = sdir > 0 ? (p.ix + 1) * a.dx : p.ix * a.dx;
+                } else if (axis == 1) {
+                    sdir = p.uy > 0.f ? 1 : -1;
+                    jy += sdir;
+                    p.y = sdir > 0 ? (p.iy + 1) * a.dy : p.iy * a.dy;
+                } else {
+                    sdir = p.uz > 0.f ? 1 : -1;
+                    jz += sdir;
+                    p.z = sdir > 0 ? (p.iz + 1) * a.dz : p.iz * a.dz;
+                }
+                bool ended = false, reflected = false;
+                if (axis == depth_axis) {
+                    const int jd = depth_axis == 2 ? jz : jx;
+                    const int nd = depth_axis == 2 ? Nz : Nx;
+                    if (jd < 0) {
+                        float &ud = depth_axis == 2 ? p.uz : p.ux;
+                        const float r_out = fresnel(p.o.w, (float)a.n_amb, ud);
+                        if (r_out > 0.f && rng_u(rng) < r_out) {
+                            ud = -ud;
+                            reflected = true;
+                        } else {
+                            t_diff += (double)p.w;
+                            ended = true;
+                        }
+                    } else if (jd >= nd) {
+                        t_trans += (double)p.w;
+                        ended = true;
+                    }
+                } else if (jx < 0 || jx >= Nx || jy < 0 || jy >= Ny || jz < 0 || jz >= Nz) {
+                    die_lateral(a.la == 0 ? p.iz : p.ix, p.w);
+                    continue;
+                }
+                if (ended) {
+                    alive = false;
+                    flush();
+                    continue;
+                }
+                if (reflected) {
+                    if (p.ev >= a.max_events) { alive = false; flush(); }
+                    continue;
+                }
+                if (a.rmax2 > 0.f) {
+                    const float rr = a.la == 0 ? (p.x - a.xc) * (p.x - a.xc) + (p.y - a.yc) * (p.y - a.yc)
+                                               : (p.y - a.yc) * (p.y - a.yc) + (p.z - a.zc) * (p.z - a.zc);
+                    if (rr >= a.rmax2) {
+                        die_lateral(a.la == 0 ? p.iz : p.ix, p.w);
+                        continue;
+                    }
+                }
+                const float4 o2 = a.opt[((int64_t)jx * Ny + jy) * Nz + jz];
+                const float n1 = p.o.w, n2 = o2.w;
+                if (fabsf(n1 - n2) > 1e-9f) {
+                    float &un = axis == 0 ? p.ux : (axis == 1 ? p.uy : p.uz);
+                    const float r_int = fresnel(n1, n2, un);
+                    if (rng_u(rng) < r_int) {
+                        un = -un;
+                        if (p.ev >= a.max_events) { alive = false; flush(); }
+                        continue;
+                    }
+                    const float eta = n1 / n2;
+                    if (axis == 0) refract(p.ux, p.uy, p.uz, eta);
+                    else if (axis == 1) refract(p.uy, p.ux, p.uz, eta);
+                    else refract(p.uz, p.ux, p.uy, eta);
+                }
+                p.ix = jx;
+                p.iy = jy;
+                p.iz = jz;
+                p.o = o2;
+            }
+            if (alive && p.ev >= a.max_events) {
+                alive = false;  // voxel_mc_3d.py:196: the event budget ends the packet untallied
+                flush();
+            }
+        }
+    }
+    flush();
+    // ---- wave reduction of the FP64 scalar tallies, one atomic per wave
+    double vals[6] = {t_spec, t_diff, t_trans, t_lat, t_abs, (double)t_ev};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double v = vals[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.acc[q], v);
+    }
+}
+
+// ------------------------------------------------------------------ optics pack
+// Transport optics per voxel: base optics plus the plume extinction sampled at
+// the voxel centre (coupled_ablation.py:445-450: mu_a += mu_a_plume,
+// mu_s += mu_s_plume, g = 0.8 where plume > 1e-3 inside the void).
+__global__ void k_pack(const float *__restrict__ mu_a, const float *__restrict__ mu_s, const float *__restrict__ g,
+                       const float *__restrict__ n, const float *__restrict__ ext, const uint8_t *__restrict__ flags,
+                       float4 *__restrict__ out, int Nx, int Ny, int Nz, int Px, int Py, int Pz, float ra, float rs,
+                       int add)
+{
+    const int64_t N = (int64_t)Nx * Ny * Nz;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        float4 o = make_float4(mu_a[i], mu_s[i], g[i], n[i]);
+        if (add) {
+            const int z = (int)(i % Nz);
+            const int64_t r = i / Nz;
+            const int y = (int)(r % Ny), x = (int)(r / Ny);
+            const int I = (int)(((int64_t)(2 * x + 1) * Px) / (2 * (int64_t)Nx));
+            const int J = (int)(((int64_t)(2 * y + 1) * Py) / (2 * (int64_t)Ny));
+            const int K = (int)(((int64_t)(2 * z + 1) * Pz) / (2 * (int64_t)Nz));
+            const float e = ext[((int64_t)I * Py + J) * Pz + K];
+            o.x += e * ra;
+            o.y += e * rs;
+            if (e > 0.001f && (flags[i] & F_ABL)) o.z = 0.80f;
+        }
+        out[i] = o;
+    }
+}
+
+int launch_pack_optics(Ctx *c, bool add_plume)
+{
+    BD_TRY(ensure(c, (void **)&c->optics, (size_t)c->nvox * sizeof(float4)));
+    const bool add = add_plume && c->plume.enabled && c->Pext && c->flags;
+    const double mu_max = c->plume.mu_max;
+    const float ra = add ? (float)(c->plume.mu_a_max / std::fmax(1e-6, mu_max)) : 0.f;
+    const float rs = add ? (float)(c->plume.mu_s_max / std::fmax(1e-6, mu_max)) : 0.f;
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_pack, dim3(nb), dim3(256), 0, c->stream, c->mu_a, c->mu_s, c->gg, c->nn, c->Pext, c->flags,
+                       c->optics, c->g.nx, c->g.ny, c->g.nz, add ? c->plume.nx : 1, add ? c->plume.ny : 1,
+                       add ? c->plume.nz : 1, ra, rs, add ? 1 : 0);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// Q = A * power / (N * V)  (voxel_mc_3d.py:684)
+__global__ void k_q_from_a(const float2 *__restrict__ AL, float *__restrict__ Q, float scale, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        Q[i] = AL[i].x * scale;
+}
+
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
+{
+    BD_TRY(launch_pack_optics(c, b->add_plume != 0));
+    BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
+    BD_TRY(ensure(c, (void **)&c->Q, (size_t)c->nvox * sizeof(float)));
+    const int64_t nd = b->launch_axis == 0 ? c->g.nz : c->g.nx;
+    if (c->lat_n != nd) {
+        if (c->lat) BD_HIP(hipFree(c->lat));
+        c->lat = nullptr;
+        BD_HIP(hipMalloc((void **)&c->lat, sizeof(double) * nd));
+        c->lat_n = nd;
+    }
+    BD_TRY(launch_fill(c, c->AL, K_F32, c->nvox * 2, 0.0));
+    BD_TRY(launch_fill(c, c->lat, K_F64, nd, 0.0));
+    BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
+    BD_TRY(launch_fill(c, c->counter, K_F64, 1, 0.0));
+
+    McArgs a{};
+    a.opt = c->optics;
+    a.AL = c->AL;
+    a.acc = c->red + 16;
+    a.lat = c->lat;
+    a.counter = c->counter;
+    a.Nx = c->g.nx;
+    a.Ny = c->g.ny;
+    a.Nz = c->g.nz;
+    a.dx = (float)c->g.dx;
+    a.dy = (float)c->g.dy;
+    a.dz = (float)c->g.dz;
+    a.Lx = (float)(c->g.dx * c->g.nx);
+    a.Ly = (float)(c->g.dy * c->g.ny);
+    a.Lz = (float)(c->g.dz * c->g.nz);
+    const double w0 = b->spot_dia / 2.0;
+    a.sigma = (float)(w0 / 2.0);
+    a.R = (float)w0;
+    a.profile = b->beam_profile;
+    a.la = b->launch_axis;
+    const double sth = std::sin(b->theta_rad), cth = std::cos(b->theta_rad);
+    const double sph = std::sin(b->phi_rad), cph = std::cos(b->phi_rad);
+    double iu[3];
+    if (b->launch_axis == 0) {
+        iu[0] = sth * cph; iu[1] = sth * sph; iu[2] = cth;
+        a.cos_in_d = cth;
+    } else {
+        iu[0] = cth; iu[1] = sth * cph; iu[2] = sth * sph;
+        a.cos_in_d = cth;
+    }
+    a.iu0 = (float)iu[0];
+    a.iu1 = (float)iu[1];
+    a.iu2 = (float)iu[2];
+    a.cos_ph = (float)cph;
+    a.sin_ph = (float)sph;
+    a.cts = (float)std::fmax(cth, 0.05);
+    a.xc = (float)b->x_center;
+    a.yc = (float)b->y_center;
+    a.zc = (float)b->z_center;
+    a.n_amb = b->n_ambient;
+    a.rmax2 = b->r_max > 0.0 ? (float)(b->r_max * b->r_max) : 0.f;
+    a.n_photons = b->n_photons;
+    a.photon_offset = b->photon_offset;
+    a.max_events = b->max_events;
+    a.k0 = (unsigned)(b->seed & 0xffffffffu);
+    a.k1 = (unsigned)(b->seed >> 32);
+    a.score_L = b->score_fluence ? 1 : 0;
+
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    const int64_t want = (b->n_photons + 255) / 256;
+    const int64_t cap = (int64_t)ncu * 8;
+    const int nblk = (int)(want < cap ? (want < 1 ? 1 : want) : cap);
+    BD_HIP(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_mc, dim3(nblk), dim3(256), 0, c->stream, a);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipEventRecord(c->ev1, c->stream));
+    const float scale = (float)(b->power / ((double)b->n_photons * c->g.dx * c->g.dy * c->g.dz));
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_q_from_a, dim3(nb), dim3(256), 0, c->stream, c->AL, c->Q, scale, c->nvox);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    const double *r = (const double *)c->host_pinned;
+    out->specular = r[0];
+    out->diffuse = r[1];
+    out->transmitted = r[2];
+    out->lateral = r[3];
+    out->absorbed = r[4];
+    out->events = r[5];
+    float ms = 0.f;
+    BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    out->kernel_ms = ms;
+    return BD_OK;
+}
+
+}  // namespace bd

@@ -1,0 +1,471 @@
+// bd_plume.hip -- plume extinction field on the device (plume.py:18-366,
+// coupled_ablation.py:587-603, 619-626).
+//
+// The plume lives on its own grid (Px, Py, Pz) covering the same box as the
+// thermal grid.  With Px,Py,Pz == Nx,Ny,Nz every mapping below is the identity
+// and the kernels are the reference's arithmetic cell for cell.  A coarser
+// plume grid (e.g. 128^3 under a 256x256x192 thermal grid, where the explicit
+// plume scheme at the thermal spacing would be unstable) is coupled by
+// cell-centre sampling: a plume cell is void when the thermal cell under its
+// centre is void, a removed thermal cell deposits delta_mu * (V_thermal /
+// V_plume) into the plume cell under its centre (mass conserving), and the
+// transport optics read the plume cell under each thermal voxel's centre.
+#include "bd_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace bd {
+
+__device__ __forceinline__ int cmap(int i, int n_from, int n_to)
+{
+    return (int)(((int64_t)(2 * i + 1) * n_to) / (2 * (int64_t)n_from));
+}
+
+// plume.py:18-80 (FP32 arithmetic)
+__global__ __launch_bounds__(256) void k_plume_diffuse(const float *__restrict__ C, float *__restrict__ Cn,
+                                                        const uint8_t *__restrict__ vm, int Nx, int Ny, int Nz,
+                                                        float idx2, float idy2, float idz2, float inv_dz, float dt,
+                                                        float Dxy, float Dz, float vz, float decay)
+{
+    const int64_t N = (int64_t)Nx * Ny * Nz;
+    const int64_t pl = (int64_t)Ny * Nz;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+        if (!vm[c]) {
+            Cn[c] = 0.f;
+            continue;
+        }
+        const int k = (int)(c % Nz);
+        const int64_t r = c / Nz;
+        const int j = (int)(r % Ny), i = (int)(r / Ny);
+        if (i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1 || k == 0 || k == Nz - 1) {
+            Cn[c] = 0.f;
+            continue;
+        }
+        const float Cc = C[c];
+        const float Cw = vm[c - pl] ? C[c - pl] : Cc;
+        const float Ce = vm[c + pl] ? C[c + pl] : Cc;
+        const float Cs = vm[c - Nz] ? C[c - Nz] : Cc;
+        const float Cq = vm[c + Nz] ? C[c + Nz] : Cc;
+        const float Cb = vm[c - 1] ? C[c - 1] : Cc;
+        const float Ct = vm[c + 1] ? C[c + 1] : Cc;
+        const float diff = Dxy * ((Cw - 2.f * Cc + Ce) * idx2 + (Cs - 2.f * Cc + Cq) * idy2) +
+                           Dz * ((Cb - 2.f * Cc + Ct) * idz2);
+        const float adv = vz > 0.f ? -vz * (Cc - Ct) * inv_dz : 0.f;
+        const float v = (Cc + dt * (diff + adv)) * decay;
+        Cn[c] = fmaxf(v, 0.f);
+    }
+}
+
+static int ew_blocks(int64_t n) { return blocks_for(n, 256) > 8192 ? 8192 : blocks_for(n, 256); }
+
+static void plume_spacing(Ctx *c, double &dx, double &dy, double &dz)
+{
+    dx = c->g.dx * c->g.nx / c->plume.nx;
+    dy = c->g.dy * c->g.ny / c->plume.ny;
+    dz = c->g.dz * c->g.nz / c->plume.nz;
+}
+
+int launch_plume_substeps(Ctx *c, int n, double dt, double decay)
+{
+    const bd_plume_params &p = c->plume;
+    double dx, dy, dz;
+    plume_spacing(c, dx, dy, dz);
+    for (int s = 0; s < n; ++s) {
+        hipLaunchKernelGGL(k_plume_diffuse, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->P2, c->pvoid,
+                           p.nx, p.ny, p.nz, (float)(1.0 / (dx * dx)), (float)(1.0 / (dy * dy)),
+                           (float)(1.0 / (dz * dz)), (float)(1.0 / dz), (float)dt, (float)p.D_xy, (float)p.D_z,
+                           (float)p.v_drift, (float)decay);
+        BD_HIP(hipGetLastError());
+        std::swap(c->P, c->P2);
+    }
+    return BD_OK;
+}
+
+// any(ablated[:, :, k0-1])  (coupled_ablation.py:624)
+__global__ void k_gap_any(const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz, int kk,
+                          unsigned long long *__restrict__ out)
+{
+    const int64_t cols = (int64_t)Nx * Ny;
+    int hit = 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < cols; c += (int64_t)gridDim.x * blockDim.x)
+        hit |= (flags[c * Nz + kk] & F_ABL) ? 1 : 0;
+    if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(out, 1ull);
+}
+
+__global__ void k_plume_void(const uint8_t *__restrict__ flags, const unsigned long long *__restrict__ breach,
+                             uint8_t *__restrict__ vm, int Nx, int Ny, int Nz, int Px, int Py, int Pz, int k0, int k1)
+{
+    const int64_t N = (int64_t)Px * Py * Pz;
+    const bool br = breach && *breach;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+        const int K = (int)(c % Pz);
+        const int64_t r = c / Pz;
+        const int J = (int)(r % Py), I = (int)(r / Py);
+        const int i = cmap(I, Px, Nx), j = cmap(J, Py, Ny), k = cmap(K, Pz, Nz);
+        const bool v = (flags[((int64_t)i * Ny + j) * Nz + k] & F_ABL) || (br && k >= k0 && k < k1);
+        vm[c] = v ? 1 : 0;
+    }
+}
+
+int launch_plume_void(Ctx *c)
+{
+    const bd_physics &ph = c->phys;
+    const bool gap = c->has_phys && ph.air_gap_k0 > 0 && ph.air_gap_k1 > ph.air_gap_k0;
+    unsigned long long *breach = nullptr;
+    if (gap) {
+        breach = c->counter + 2;
+        BD_TRY(launch_fill(c, breach, K_F64, 1, 0.0));
+        const int64_t cols = (int64_t)c->g.nx * c->g.ny;
+        hipLaunchKernelGGL(k_gap_any, dim3(blocks_for(cols, 256) > 1024 ? 1024 : blocks_for(cols, 256)), dim3(256), 0,
+                           c->stream, c->flags, c->g.nx, c->g.ny, c->g.nz, ph.air_gap_k0 - 1, breach);
+        BD_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_plume_void, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->flags, breach, c->pvoid,
+                       c->g.nx, c->g.ny, c->g.nz, c->plume.nx, c->plume.ny, c->plume.nz,
+                       gap ? ph.air_gap_k0 : 0, gap ? ph.air_gap_k1 : 0);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+__global__ void k_inject_point(const uint8_t *__restrict__ rem, float *__restrict__ P, int Nx, int Ny, int Nz,
+                               int Px, int Py, int Pz, float amount, int same)
+{
+    const int64_t N = (int64_t)Nx * Ny * Nz;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+        if (!rem[c]) continue;
+        if (same) {
+            P[c] += amount;  // plume.py:329 raw_field[newly] += delta_mu
+        } else {
+            const int k = (int)(c % Nz);
+            const int64_t r = c / Nz;
+            const int j = (int)(r % Ny), i = (int)(r / Ny);
+            const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
+            unsafeAtomicAdd(&P[pc], amount);
+        }
+    }
+}
+
+__global__ void k_mask_void(float *__restrict__ P, const uint8_t *__restrict__ vm, int64_t n)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+        if (!vm[c]) P[c] = 0.f;
+}
+
+// bounding box of the removed cells: slots {xmin, xmax, ymin, ymax, zmin, zmax}
+__global__ void k_bbox(const uint8_t *__restrict__ rem, int Nx, int Ny, int Nz, unsigned long long *__restrict__ bb)
+{
+    const int64_t N = (int64_t)Nx * Ny * Nz;
+    int mn[3] = {1 << 30, 1 << 30, 1 << 30}, mx[3] = {-1, -1, -1};
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+        if (!rem[c]) continue;
+        const int k = (int)(c % Nz);
+        const int64_t r = c / Nz;
+        const int j = (int)(r % Ny), i = (int)(r / Ny);
+        mn[0] = min(mn[0], i); mx[0] = max(mx[0], i);
+        mn[1] = min(mn[1], j); mx[1] = max(mx[1], j);
+        mn[2] = min(mn[2], k); mx[2] = max(mx[2], k);
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (mx[a] >= 0) {
+            atomicMin(&bb[2 * a], (unsigned long long)mn[a]);
+            atomicMax(&bb[2 * a + 1], (unsigned long long)mx[a]);
+        }
+    }
+}
+
+// plume.py:105-114 as a gather: t1[ox][y][z] = sum_a imp[ox-a][y][z] * kx[a] (a ascending)
+// impulse source: the removed-cell mask at box origin (xm, ym, zm), or an
+// explicit float32 box when `box` is not null
+__global__ void k_splat1(const uint8_t *__restrict__ rem, const float *__restrict__ box, int Ny, int Nz, int xm,
+                         int ym, int zm, int Lx, int Ly, int Lz, const float *__restrict__ kx, int nkx,
+                         float *__restrict__ t1)
+{
+    const int OX = Lx + nkx - 1;
+    const int64_t N = (int64_t)OX * Ly * Lz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(t % Lz);
+        const int64_t r = t / Lz;
+        const int y = (int)(r % Ly), ox = (int)(r / Ly);
+        float s = 0.f;
+        for (int a = 0; a < nkx; ++a) {
+            const int ix = ox - a;
+            if (ix < 0 || ix >= Lx) continue;
+            const float imp = box ? box[((int64_t)ix * Ly + y) * Lz + z]
+                                  : (rem[((int64_t)(xm + ix) * Ny + (ym + y)) * Nz + (zm + z)] ? 1.f : 0.f);
+            s += imp * kx[a];
+        }
+        t1[t] = s;
+    }
+}
+
+// plume.py:116-125: t2[ox][oy][z] = sum_b t1[ox][oy-b][z] * ky[b]
+__global__ void k_splat2(const float *__restrict__ t1, int OX, int Ly, int Lz, const float *__restrict__ ky, int nky,
+                         float *__restrict__ t2)
+{
+    const int OY = Ly + nky - 1;
+    const int64_t N = (int64_t)OX * OY * Lz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(t % Lz);
+        const int64_t r = t / Lz;
+        const int oy = (int)(r % OY), ox = (int)(r / OY);
+        float s = 0.f;
+        for (int b = 0; b < nky; ++b) {
+            const int iy = oy - b;
+            if (iy < 0 || iy >= Ly) continue;
+            s += t1[((int64_t)ox * Ly + iy) * Lz + z] * ky[b];
+        }
+        t2[t] = s;
+    }
+}
+
+// plume.py:127-139: raw[gx][gy][gz] += t2 * (kz[c] * delta_mu), c ascending, FP64
+// products rounded to FP32 after each tap exactly as the reference's typing does
+__global__ void k_splat3(const float *__restrict__ t2, int OX, int OY, int Lz, const float *__restrict__ kz, int nkz,
+                         double delta, int x0, int y0, int z0, float *__restrict__ P, int Nx, int Ny, int Nz)
+{
+    const int OZ = Lz + nkz - 1;
+    const int64_t N = (int64_t)OX * OY * OZ;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x) {
+        const int oz = (int)(t % OZ);
+        const int64_t r = t / OZ;
+        const int oy = (int)(r % OY), ox = (int)(r / OY);
+        const int gx = x0 + ox, gy = y0 + oy, gz = z0 + oz;
+        if (gx < 0 || gx >= Nx || gy < 0 || gy >= Ny || gz < 0 || gz >= Nz) continue;
+        float *dst = &P[((int64_t)gx * Ny + gy) * Nz + gz];
+        float v = *dst;
+        for (int cc = 0; cc < nkz; ++cc) {
+            const int iz = oz - cc;
+            if (iz < 0 || iz >= Lz) continue;
+            const double w = (double)kz[cc] * delta;
+            v = (float)((double)v + (double)t2[((int64_t)ox * OY + oy) * Lz + iz] * w);
+        }
+        *dst = v;
+    }
+}
+
+// three separable passes of an impulse box into the plume raw field; origin of
+// the output box is (xo, yo, zo) in plume-grid cells
+static int splat_box(Ctx *c, float *const *k3, const int *nk, const float *box, int xm, int ym, int zm, int Lx,
+                     int Ly, int Lz, int xo, int yo, int zo, double delta)
+{
+    const int OX = Lx + nk[0] - 1, OY = Ly + nk[1] - 1, OZ = Lz + nk[2] - 1;
+    const size_t need = std::max((size_t)OX * Ly * Lz, (size_t)OX * OY * Lz) * sizeof(float);
+    if (c->splat_tmp_bytes < need) {
+        if (c->splat_tmp1) BD_HIP(hipFree(c->splat_tmp1));
+        if (c->splat_tmp2) BD_HIP(hipFree(c->splat_tmp2));
+        c->splat_tmp1 = c->splat_tmp2 = nullptr;
+        BD_HIP(hipMalloc((void **)&c->splat_tmp1, need));
+        BD_HIP(hipMalloc((void **)&c->splat_tmp2, need));
+        c->splat_tmp_bytes = need;
+    }
+    const int64_t n1 = (int64_t)OX * Ly * Lz, n2 = (int64_t)OX * OY * Lz, n3 = (int64_t)OX * OY * OZ;
+    hipLaunchKernelGGL(k_splat1, dim3(ew_blocks(n1)), dim3(256), 0, c->stream, c->to_remove, box, c->g.ny, c->g.nz,
+                       xm, ym, zm, Lx, Ly, Lz, k3[0], nk[0], c->splat_tmp1);
+    hipLaunchKernelGGL(k_splat2, dim3(ew_blocks(n2)), dim3(256), 0, c->stream, c->splat_tmp1, OX, Ly, Lz, k3[1], nk[1],
+                       c->splat_tmp2);
+    hipLaunchKernelGGL(k_splat3, dim3(ew_blocks(n3)), dim3(256), 0, c->stream, c->splat_tmp2, OX, OY, Lz, k3[2], nk[2],
+                       delta, xo, yo, zo, c->P, c->plume.nx, c->plume.ny, c->plume.nz);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+static int splat(Ctx *c, float *const *k3, const int *nk, const int *rr)
+{
+    if (!k3[0] || !k3[1] || !k3[2]) {
+        set_error("plume splat kernels not uploaded (bd_set_plume_kernels)");
+        return BD_ESTATE;
+    }
+    unsigned long long *bb = c->counter + 4;
+    unsigned long long init[6] = {~0ull, 0ull, ~0ull, 0ull, ~0ull, 0ull};
+    BD_HIP(hipMemcpyAsync(bb, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_bbox, dim3(ew_blocks(c->nvox)), dim3(256), 0, c->stream, c->to_remove, c->g.nx, c->g.ny,
+                       c->g.nz, bb);
+    BD_HIP(hipGetLastError());
+    unsigned long long h[6];
+    BD_HIP(hipMemcpyAsync(h, bb, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    if (h[0] == ~0ull) return BD_OK;  // nothing removed
+    const int xm = (int)h[0], ym = (int)h[2], zm = (int)h[4];
+    const int Lx = (int)h[1] - xm + 1, Ly = (int)h[3] - ym + 1, Lz = (int)h[5] - zm + 1;
+    return splat_box(c, k3, nk, nullptr, xm, ym, zm, Lx, Ly, Lz, xm - rr[0], ym - rr[1], zm - rr[2],
+                     c->plume.delta_mu);
+}
+
+int launch_plume_splat_box(Ctx *c, int which, const float *host_box, int Lx, int Ly, int Lz, int x0, int y0, int z0,
+                           double delta)
+{
+    float *const *k3 = which == 0 ? c->kb : c->kl;
+    const int *nk = which == 0 ? c->nkb : c->nkl;
+    if (!k3[0] || !k3[1] || !k3[2]) {
+        set_error("plume splat kernels not uploaded (bd_set_plume_kernels)");
+        return BD_ESTATE;
+    }
+    const size_t bytes = (size_t)Lx * Ly * Lz * sizeof(float);
+    BD_TRY(ensure_staging(c, bytes < 64 ? 64 : bytes));
+    BD_HIP(hipMemcpyAsync(c->staging, host_box, bytes, hipMemcpyHostToDevice, c->stream));
+    BD_TRY(splat_box(c, k3, nk, (const float *)c->staging, 0, 0, 0, Lx, Ly, Lz, x0, y0, z0, delta));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    return BD_OK;
+}
+
+// plume.py:292-331: injection of the TO_REMOVE cells, then confinement to the
+// current VOID mask (the engine computes it first, coupled_ablation.py:657-659)
+int launch_plume_inject(Ctx *c)
+{
+    const bd_plume_params &p = c->plume;
+    const bool same = p.nx == c->g.nx && p.ny == c->g.ny && p.nz == c->g.nz;
+    if (p.blast_expansion) {
+        BD_TRY(splat(c, c->kb, c->nkb, c->rb));
+    } else {
+        const double ratio = same ? 1.0 : (double)c->pvox / (double)c->nvox;
+        hipLaunchKernelGGL(k_inject_point, dim3(ew_blocks(c->nvox)), dim3(256), 0, c->stream, c->to_remove, c->P,
+                           c->g.nx, c->g.ny, c->g.nz, p.nx, p.ny, p.nz, (float)(p.delta_mu * ratio), same ? 1 : 0);
+        BD_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mask_void, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->pvoid, c->pvox);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// plume.py:352-366
+__global__ void k_plume_sync(float *__restrict__ P, float *__restrict__ E, const uint8_t *__restrict__ vm, float mu_max,
+                             int64_t n)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        if (!vm[c]) {
+            P[c] = 0.f;
+            E[c] = 0.f;
+            continue;
+        }
+        const float r = P[c];
+        E[c] = mu_max > 0.f ? tanhf(r / mu_max) * mu_max : r;
+    }
+}
+
+int launch_plume_sync(Ctx *c)
+{
+    hipLaunchKernelGGL(k_plume_sync, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->Pext, c->pvoid,
+                       (float)c->plume.mu_max, c->pvox);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// legacy separable path (plume.py:374-432)
+__global__ void k_any_nonzero(const float *__restrict__ a, int64_t n, unsigned long long *__restrict__ out)
+{
+    int hit = 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+        hit |= a[c] != 0.f;
+    if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(out, 1ull);
+}
+
+__global__ void k_legacy_pre(float *__restrict__ P, const float *__restrict__ E,
+                             const unsigned long long *__restrict__ flags2, float decay, int do_decay, int64_t n)
+{
+    const bool copy = flags2[0] == 0 && flags2[1] != 0;  // raw all zero and ext not
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        float v = copy ? E[c] : P[c];
+        if (do_decay) v *= decay;
+        P[c] = v;
+    }
+}
+
+__global__ void k_legacy_post(float *__restrict__ P, float *__restrict__ E, const uint8_t *__restrict__ vm,
+                              float mu_max, int64_t n)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const float r = P[c] * (vm[c] ? 1.f : 0.f);
+        P[c] = r;
+        E[c] = mu_max > 0.f ? tanhf(r / mu_max) * mu_max : r;
+    }
+}
+
+int launch_plume_update_sep(Ctx *c, double dt_epoch)
+{
+    const bd_plume_params &p = c->plume;
+    unsigned long long *fl = c->counter + 10;
+    BD_TRY(launch_fill(c, fl, K_F64, 2, 0.0));
+    hipLaunchKernelGGL(k_any_nonzero, dim3(256), dim3(256), 0, c->stream, c->P, c->pvox, fl);
+    hipLaunchKernelGGL(k_any_nonzero, dim3(256), dim3(256), 0, c->stream, c->Pext, c->pvox, fl + 1);
+    const bool do_decay = p.tau_linger > 0.0 && dt_epoch > 0.0;
+    const float decay = do_decay ? (float)std::exp(-dt_epoch / p.tau_linger) : 1.f;
+    hipLaunchKernelGGL(k_legacy_pre, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->Pext, fl, decay,
+                       do_decay ? 1 : 0, c->pvox);
+    BD_HIP(hipGetLastError());
+    BD_TRY(splat(c, c->kl, c->nkl, c->rl));
+    hipLaunchKernelGGL(k_legacy_post, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->Pext, c->pvoid,
+                       (float)p.mu_max, c->pvox);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// coupled_ablation.py:594-603: centre-column optical depth, peak, mean over ablated cells
+__global__ void k_plume_diag(const float *__restrict__ E, const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz,
+                             int Px, int Py, int Pz, float ra, int I, int J, double dzp, double *__restrict__ acc,
+                             unsigned *__restrict__ peak_bits)
+{
+    // acc[0] column tau, acc[1] sum over ablated, acc[2] count ablated
+    const int64_t NP = (int64_t)Px * Py * Pz;
+    const int64_t N = (int64_t)Nx * Ny * Nz;
+    double col = 0.0, s = 0.0, cnt = 0.0;
+    unsigned pk = 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < NP; c += (int64_t)gridDim.x * blockDim.x) {
+        const float m = E[c] * ra;
+        pk = max(pk, __float_as_uint(m));
+        const int64_t r = c / Pz;
+        if ((int)(r % Py) == J && (int)(r / Py) == I) col += (double)m * dzp;
+    }
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+        if (!(flags[c] & F_ABL)) continue;
+        const int k = (int)(c % Nz);
+        const int64_t r = c / Nz;
+        const int j = (int)(r % Ny), i = (int)(r / Ny);
+        const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
+        s += (double)(E[pc] * ra);
+        cnt += 1.0;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        col += __shfl_down(col, off);
+        s += __shfl_down(s, off);
+        cnt += __shfl_down(cnt, off);
+        pk = max(pk, (unsigned)__shfl_down((int)pk, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (col != 0.0) unsafeAtomicAdd(&acc[0], col);
+        if (s != 0.0) unsafeAtomicAdd(&acc[1], s);
+        if (cnt != 0.0) unsafeAtomicAdd(&acc[2], cnt);
+        if (pk) atomicMax(peak_bits, pk);
+    }
+}
+
+int launch_plume_diag(Ctx *c, double out[4])
+{
+    const bd_plume_params &p = c->plume;
+    double *acc = c->red + 24;
+    unsigned *pk = (unsigned *)(c->red + 27);
+    BD_TRY(launch_fill(c, acc, K_F64, 4, 0.0));
+    double dx, dy, dz;
+    plume_spacing(c, dx, dy, dz);
+    const int I = (int)(((int64_t)(2 * (c->g.nx / 2) + 1) * p.nx) / (2 * (int64_t)c->g.nx));
+    const int J = (int)(((int64_t)(2 * (c->g.ny / 2) + 1) * p.ny) / (2 * (int64_t)c->g.ny));
+    const float ra = (float)(p.mu_a_max / std::fmax(1e-6, p.mu_max));
+    hipLaunchKernelGGL(k_plume_diag, dim3(ew_blocks(std::max(c->pvox, c->nvox))), dim3(256), 0, c->stream, c->Pext,
+                       c->flags, c->g.nx, c->g.ny, c->g.nz, p.nx, p.ny, p.nz, ra, I, J, dz, acc, pk);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipMemcpyAsync(c->host_pinned, acc, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    const double *h = (const double *)c->host_pinned;
+    unsigned bits;
+    std::memcpy(&bits, (const char *)c->host_pinned + 24, 4);
+    float peak;
+    std::memcpy(&peak, &bits, 4);
+    out[0] = h[0];
+    out[1] = std::exp(-h[0]);
+    out[2] = (double)peak;
+    out[3] = h[2] > 0.0 ? h[1] / h[2] : 0.0;
+    return BD_OK;
+}
+
+}  // namespace bd

@@ -1,0 +1,296 @@
+// bd_util.hip -- element-wise helpers: dtype conversion, fills, deterministic
+// FP64 reductions, slices and the phase map.  All are HBM-bound single passes.
+#include "bd_internal.hpp"
+
+#include <cstring>
+
+namespace bd {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+
+int ensure(Ctx *c, void **p, size_t bytes)
+{
+    if (*p) return BD_OK;
+    BD_HIP(hipMalloc(p, bytes));
+    BD_HIP(hipMemsetAsync(*p, 0, bytes, c->stream));
+    return BD_OK;
+}
+
+int ensure_staging(Ctx *c, size_t bytes)
+{
+    if (c->staging_bytes >= bytes) return BD_OK;
+    if (c->staging) BD_HIP(hipFree(c->staging));
+    c->staging = nullptr;
+    BD_HIP(hipMalloc(&c->staging, bytes));
+    c->staging_bytes = bytes;
+    return BD_OK;
+}
+
+int ensure_state(Ctx *c)
+{
+    const size_t f = (size_t)c->nvox * sizeof(float);
+    BD_TRY(ensure(c, (void **)&c->T, f));
+    BD_TRY(ensure(c, (void **)&c->T2, f));
+    BD_TRY(ensure(c, (void **)&c->H, f));
+    BD_TRY(ensure(c, (void **)&c->chi, f));
+    BD_TRY(ensure(c, (void **)&c->Q, f));
+    BD_TRY(ensure(c, (void **)&c->k, f));
+    BD_TRY(ensure(c, (void **)&c->rc, f));
+    BD_TRY(ensure(c, (void **)&c->flags, (size_t)c->nvox));
+    return BD_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = (float)s[i];
+}
+__global__ void k_f32_to_f64(const float *__restrict__ s, double *__restrict__ d, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = (double)s[i];
+}
+// set / clear one flag bit from a u8 mask
+__global__ void k_mask_to_flag(const uint8_t *__restrict__ m, uint8_t *__restrict__ f, uint8_t bit, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        f[i] = m[i] ? (uint8_t)(f[i] | bit) : (uint8_t)(f[i] & ~bit);
+}
+__global__ void k_flag_to_mask(const uint8_t *__restrict__ f, uint8_t *__restrict__ m, uint8_t bit, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m[i] = (f[i] & bit) ? 1 : 0;
+}
+__global__ void k_hpos_from_h(const float *__restrict__ H, uint8_t *__restrict__ f, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        f[i] = H[i] > 0.0f ? (uint8_t)(f[i] | F_HPOS) : (uint8_t)(f[i] & ~F_HPOS);
+}
+__global__ void k_al_get(const float2 *__restrict__ al, float *__restrict__ d, int which, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = which ? al[i].y : al[i].x;
+}
+__global__ void k_al_set(float2 *__restrict__ al, const float *__restrict__ s, int which, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (which) al[i].y = s[i];
+        else al[i].x = s[i];
+    }
+}
+__global__ void k_fill_f32(float *d, float v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = v;
+}
+__global__ void k_fill_f64(double *d, double v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = v;
+}
+__global__ void k_fill_u8(uint8_t *d, uint8_t v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = v;
+}
+
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+template <int KIND>
+__device__ inline double load_as_double(const void *p, int64_t i)
+{
+    if (KIND == K_F32) return (double)((const float *)p)[i];
+    if (KIND == K_F64) return ((const double *)p)[i];
+    return (double)((const uint8_t *)p)[i];
+}
+
+// op 0 sum, 1 max, 2 min.  Fixed grid, one partial per block, then one final
+// block: the FP64 result is bit-reproducible run to run.
+template <int KIND>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const void *__restrict__ src, int64_t n, int op,
+                                                                 const uint8_t *__restrict__ mask, uint8_t bits,
+                                                                 int invert, double *__restrict__ partial)
+{
+    __shared__ double s[RED_THREADS];
+    double acc = op == 0 ? 0.0 : (op == 1 ? -1.0e308 : 1.0e308);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (mask) {
+            const bool m = (mask[i] & bits) != 0;
+            if (m == (invert != 0)) continue;
+        }
+        const double v = load_as_double<KIND>(src, i);
+        acc = op == 0 ? acc + v : (op == 1 ? fmax(acc, v) : fmin(acc, v));
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double o = s[threadIdx.x + w];
+            s[threadIdx.x] = op == 0 ? s[threadIdx.x] + o : (op == 1 ? fmax(s[threadIdx.x], o) : fmin(s[threadIdx.x], o));
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ partial, int np, int op,
+                                                               double *__restrict__ out)
+{
+    __shared__ double s[RED_THREADS];
+    double acc = op == 0 ? 0.0 : (op == 1 ? -1.0e308 : 1.0e308);
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        const double v = partial[i];
+        acc = op == 0 ? acc + v : (op == 1 ? fmax(acc, v) : fmin(acc, v));
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double o = s[threadIdx.x + w];
+            s[threadIdx.x] = op == 0 ? s[threadIdx.x] + o : (op == 1 ? fmax(s[threadIdx.x], o) : fmin(s[threadIdx.x], o));
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+static int reduce_impl(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t bits,
+                       int invert, double *out)
+{
+    // partials live in the staging tail of the reduction scratch
+    double *partial = nullptr;
+    BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + RED_BLOCKS)));
+    partial = c->red + 64;
+    dim3 gr(RED_BLOCKS), bl(RED_THREADS);
+    switch (kind) {
+    case K_F32: hipLaunchKernelGGL(k_reduce_partial<K_F32>, gr, bl, 0, c->stream, src, n, op, mask, bits, invert, partial); break;
+    case K_F64: hipLaunchKernelGGL(k_reduce_partial<K_F64>, gr, bl, 0, c->stream, src, n, op, mask, bits, invert, partial); break;
+    default: hipLaunchKernelGGL(k_reduce_partial<K_U8>, gr, bl, 0, c->stream, src, n, op, mask, bits, invert, partial); break;
+    }
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), bl, 0, c->stream, partial, RED_BLOCKS, op, c->red);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    *out = *(double *)c->host_pinned;
+    return BD_OK;
+}
+
+int launch_reduce(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t bits,
+                  double *out)
+{
+    return reduce_impl(c, src, kind, n, op, mask, bits, 0, out);
+}
+
+int reduce_masked(Ctx *c, const float *src, int64_t n, const uint8_t *flags, uint8_t bit, bool invert, double *out)
+{
+    return reduce_impl(c, src, K_F32, n, 0, flags, bit, invert ? 1 : 0, out);
+}
+
+int launch_fill(Ctx *c, void *dst, int kind, int64_t n, double v)
+{
+    const int nb = blocks_for(n, 256) > 4096 ? 4096 : blocks_for(n, 256);
+    if (kind == K_F32) hipLaunchKernelGGL(k_fill_f32, dim3(nb), dim3(256), 0, c->stream, (float *)dst, (float)v, n);
+    else if (kind == K_F64) hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(256), 0, c->stream, (double *)dst, v, n);
+    else hipLaunchKernelGGL(k_fill_u8, dim3(nb), dim3(256), 0, c->stream, (uint8_t *)dst, (uint8_t)v, n);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+
+__global__ void k_slice(const void *__restrict__ src, int kind, int Nx, int Ny, int Nz, int plane, int index,
+                        int stride, float *__restrict__ out)
+{
+    // plane 1: x-z plane at j = index -> out[(i/stride) * nzs + k/stride]
+    // plane 0: y-z plane at i = index -> out[(j/stride) * nzs + k/stride]
+    const int nzs = (Nz + stride - 1) / stride;
+    const int na = plane == 1 ? (Nx + stride - 1) / stride : (Ny + stride - 1) / stride;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na * nzs; t += gridDim.x * blockDim.x) {
+        const int a = (t / nzs) * stride, kk = (t % nzs) * stride;
+        const int64_t id = plane == 1 ? ((int64_t)a * Ny + index) * Nz + kk : ((int64_t)index * Ny + a) * Nz + kk;
+        float v;
+        if (kind == K_F32) v = ((const float *)src)[id];
+        else if (kind == K_F64) v = (float)((const double *)src)[id];
+        else if (kind == K_U8) v = (float)((const uint8_t *)src)[id];
+        else if (kind == K_FLAG_ABL) v = (((const uint8_t *)src)[id] & F_ABL) ? 1.0f : 0.0f;
+        else if (kind == K_FLAG_DRY) v = (((const uint8_t *)src)[id] & F_DRY) ? 1.0f : 0.0f;
+        else if (kind == K_AL_A) v = ((const float2 *)src)[id].x;
+        else v = ((const float2 *)src)[id].y;
+        out[t] = v;
+    }
+}
+
+int launch_slice(Ctx *c, const void *src, int kind, int plane, int index, int stride, float *out, int64_t n)
+{
+    hipLaunchKernelGGL(k_slice, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, src, kind, c->g.nx, c->g.ny,
+                       c->g.nz, plane, index, stride, out);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// run_multispectral_matrix.py:168-192: 0 intact, 1 char, 2 boiling, 3 void
+__global__ void k_phase_map(const float *__restrict__ T, const float *__restrict__ chi, const float *__restrict__ H,
+                            const uint8_t *__restrict__ f, const uint8_t *__restrict__ gap, uint8_t *__restrict__ out,
+                            int64_t n, int has_chi)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool g = gap && gap[i];
+        const uint8_t fl = f[i];
+        uint8_t ph = 0;
+        const bool boiling = ((T[i] >= 99.5f) || (H[i] > 0.0f)) && !g;
+        const bool charred = (fl & F_DRY) && has_chi && ((double)chi[i] >= 0.10) && !g;
+        if (boiling) ph = 2;
+        if (charred) ph = 1;
+        if ((fl & F_ABL) || g) ph = 3;
+        out[i] = ph;
+    }
+}
+
+int launch_phase_map(Ctx *c, const uint8_t *gap, uint8_t *out)
+{
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_phase_map, dim3(nb), dim3(256), 0, c->stream, c->T, c->chi, c->H, c->flags, gap, out,
+                       c->nvox, 1);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// exported for the API unit
+void convert_f64_to_f32(Ctx *c, const double *s, float *d, int64_t n)
+{
+    const int nb = blocks_for(n, 256) > 8192 ? 8192 : blocks_for(n, 256);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nb), dim3(256), 0, c->stream, s, d, n);
+}
+void convert_f32_to_f64(Ctx *c, const float *s, double *d, int64_t n)
+{
+    const int nb = blocks_for(n, 256) > 8192 ? 8192 : blocks_for(n, 256);
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(nb), dim3(256), 0, c->stream, s, d, n);
+}
+void mask_to_flag(Ctx *c, const uint8_t *m, uint8_t bit)
+{
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_mask_to_flag, dim3(nb), dim3(256), 0, c->stream, m, c->flags, bit, c->nvox);
+}
+void flag_to_mask(Ctx *c, uint8_t *m, uint8_t bit)
+{
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_flag_to_mask, dim3(nb), dim3(256), 0, c->stream, c->flags, m, bit, c->nvox);
+}
+void hpos_from_h(Ctx *c)
+{
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_hpos_from_h, dim3(nb), dim3(256), 0, c->stream, c->H, c->flags, c->nvox);
+}
+void al_get(Ctx *c, float *d, int which)
+{
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_al_get, dim3(nb), dim3(256), 0, c->stream, c->AL, d, which, c->nvox);
+}
+void al_set(Ctx *c, const float *s, int which)
+{
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_al_set, dim3(nb), dim3(256), 0, c->stream, c->AL, s, which, c->nvox);
+}
+
+}  // namespace bd

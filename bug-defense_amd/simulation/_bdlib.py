@@ -1,0 +1,365 @@
+"""ctypes binding of libbugdef.so (include/bugdef.h), the gfx950 engine library.
+
+There is no fallback: if the library or a HIP device is missing, the calls
+raise ``RuntimeError`` (the reference's own behaviour when a GPU engine is
+forced, coupled_ablation.py:392-396, voxel_mc_3d.py:563-565).  Status codes map
+to exceptions as BD_EINVAL -> ValueError, everything else -> RuntimeError.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("BUGDEF_LIB", os.path.join(os.path.dirname(_HERE), "lib", "libbugdef.so"))
+
+BD_OK, BD_EINVAL, BD_ENODEV, BD_EHIP, BD_ESTATE = 0, -1, -2, -3, -4
+BD_F32, BD_F64, BD_U8 = 0, 1, 2
+
+FIELDS = [
+    "T", "H_ACC", "OMEGA", "CHI", "Q", "K", "RHO_CP", "H_LAT", "DRY", "ABLATED",
+    "MU_A", "MU_S", "G", "N", "WET_MU_A", "WET_K", "WET_RHO_CP", "T_PEAK", "ABLATION_TIME",
+    "A", "L", "PLUME_RAW", "PLUME_EXT", "VOID", "TO_REMOVE",
+    "DS_MU_A", "DS_MU_S", "DS_G", "DS_N", "DS_K", "DS_RHO_CP", "DS_T_REMOVE",
+    "CS_MU_A", "CS_MU_S", "CS_K", "CS_RHO_CP", "CS_T_REMOVE",
+]
+F = {name: i for i, name in enumerate(FIELDS)}
+MASK_FIELDS = {"DRY", "ABLATED", "VOID", "TO_REMOVE"}
+PLUME_FIELDS = {"PLUME_RAW", "PLUME_EXT", "VOID"}
+
+SUB_PLATEAU, SUB_ARRHENIUS, SUB_CHAR, SUB_MASK_Q, SUB_PEAK, SUB_PLUME = 1, 2, 4, 8, 16, 32
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [
+        ("dt_sub", ctypes.c_double), ("h_eff", ctypes.c_double), ("t_ambient", ctypes.c_double),
+        ("t_abl", ctypes.c_double), ("t_void", ctypes.c_double),
+        ("has_arrhenius", ctypes.c_int32), ("arr_A", ctypes.c_double), ("arr_Ea_over_R", ctypes.c_double),
+        ("has_char", ctypes.c_int32), ("char_A", ctypes.c_double), ("char_Ea_over_R", ctypes.c_double),
+        ("phase_mode", ctypes.c_int32),
+        ("has_cs", ctypes.c_int32), ("has_cs_mu_a", ctypes.c_int32), ("has_cs_mu_s", ctypes.c_int32),
+        ("has_cs_k", ctypes.c_int32), ("has_cs_rho_cp", ctypes.c_int32),
+        ("cs_chi_remove", ctypes.c_double), ("alpha_max_stable", ctypes.c_double),
+        ("k_void", ctypes.c_double), ("rho_cp_void", ctypes.c_double),
+        ("air_gap_k0", ctypes.c_int32), ("air_gap_k1", ctypes.c_int32),
+        ("no_removal", ctypes.c_int32),
+    ]
+
+
+class PlumeParams(ctypes.Structure):
+    _fields_ = [
+        ("enabled", ctypes.c_int32), ("model", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("D_xy", ctypes.c_double), ("D_z", ctypes.c_double), ("v_drift", ctypes.c_double),
+        ("tau_linger", ctypes.c_double), ("mu_max", ctypes.c_double), ("mu_a_max", ctypes.c_double),
+        ("mu_s_max", ctypes.c_double), ("delta_mu", ctypes.c_double), ("blast_expansion", ctypes.c_int32),
+    ]
+
+
+class Beam(ctypes.Structure):
+    _fields_ = [
+        ("power", ctypes.c_double), ("spot_dia", ctypes.c_double), ("theta_rad", ctypes.c_double),
+        ("phi_rad", ctypes.c_double), ("x_center", ctypes.c_double), ("y_center", ctypes.c_double),
+        ("z_center", ctypes.c_double), ("n_ambient", ctypes.c_double), ("r_max", ctypes.c_double),
+        ("beam_profile", ctypes.c_int32), ("launch_axis", ctypes.c_int32),
+        ("n_photons", ctypes.c_int64), ("photon_offset", ctypes.c_int64), ("max_events", ctypes.c_int64),
+        ("seed", ctypes.c_uint64), ("score_fluence", ctypes.c_int32), ("add_plume", ctypes.c_int32),
+    ]
+
+
+class MCResult(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("specular", "diffuse", "transmitted", "lateral", "absorbed", "events", "kernel_ms")]
+
+
+class EpochDiag(ctypes.Structure):
+    _fields_ = [("n_new", ctypes.c_int64)] + [(n, ctypes.c_double) for n in (
+        "absorbed_frac", "mc_energy_closure", "plume_tau", "plume_transmission", "plume_peak_mu_a",
+        "plume_mean_mu_a", "t_optical_ms", "t_thermal_ms", "t_phase_ms", "absorbed_J", "reflected_J",
+        "transmitted_J", "lateral_J", "conducted_J", "ablated_J", "R_spec", "R_diff", "transmission",
+        "lateral_loss", "mc_events", "t_substeps_ms", "t_mc_kernel_ms")]
+
+
+_lib = None
+_lock = threading.Lock()
+
+_SIGS = {
+    "bd_last_error": ([], ctypes.c_char_p),
+    "bd_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "bd_version": ([ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "bd_create": ([ctypes.POINTER(Grid), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "bd_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_set_physics": ([ctypes.c_void_p, ctypes.POINTER(Physics)], ctypes.c_int),
+    "bd_set_plume": ([ctypes.c_void_p, ctypes.POINTER(PlumeParams)], ctypes.c_int),
+    "bd_set_plume_kernels": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int], ctypes.c_int),
+    "bd_upload": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_download": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_fill": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "bd_download_slice": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_substeps": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "bd_plateau": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_arrhenius": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "bd_char": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "bd_plume_substeps": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "bd_mc_run": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(MCResult)], ctypes.c_int),
+    "bd_mc_lateral": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_phase_update": ([ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "bd_plume_void": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_plume_inject": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_plume_sync": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_plume_update_separable": ([ctypes.c_void_p, ctypes.c_double], ctypes.c_int),
+    "bd_plume_splat": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "bd_plume_diag": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bd_epoch": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double,
+                  ctypes.POINTER(EpochDiag)], ctypes.c_int),
+    "bd_reduce": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                  ctypes.c_int),
+    "bd_phase_map": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_sync": ([ctypes.c_void_p], ctypes.c_int),
+}
+EXPORTS = tuple(_SIGS)
+
+
+def lib():
+    """Load libbugdef.so (raises RuntimeError if it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"libbugdef.so not found at {LIB_PATH}: build it with "
+                                   "`make -C bug-defense_amd/csrc` (no CPU fallback exists)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (args, res) in _SIGS.items():
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = res
+            _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    lib().bd_device_count(ctypes.byref(n))
+    return int(n.value)
+
+
+def is_gpu_available() -> bool:
+    try:
+        return device_count() > 0
+    except (RuntimeError, OSError):
+        return False
+
+
+def check(status: int, what: str = ""):
+    if status == BD_OK:
+        return
+    msg = lib().bd_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if status == BD_EINVAL:
+        raise ValueError(text)
+    raise RuntimeError(text)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One device context (bd_ctx): grid-sized device fields on one HIP stream."""
+
+    def __init__(self, shape, spacing):
+        nx, ny, nz = (int(v) for v in shape)
+        self.shape = (nx, ny, nz)
+        self.spacing = tuple(float(v) for v in spacing)
+        self.nvox = nx * ny * nz
+        self.plume_shape = None
+        h = ctypes.c_void_p()
+        g = Grid(nx, ny, nz, *self.spacing)
+        check(lib().bd_create(ctypes.byref(g), ctypes.byref(h)), "bd_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("context destroyed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.bd_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- configuration
+    def set_physics(self, phys: Physics):
+        check(lib().bd_set_physics(self.handle, ctypes.byref(phys)), "bd_set_physics")
+
+    def set_plume(self, pp: PlumeParams):
+        check(lib().bd_set_plume(self.handle, ctypes.byref(pp)), "bd_set_plume")
+        if pp.enabled:
+            self.plume_shape = (pp.nx, pp.ny, pp.nz)
+
+    def set_plume_kernels(self, which, kx, ky, kz, rx, ry, rz):
+        kx = np.ascontiguousarray(kx, dtype=np.float32)
+        ky = np.ascontiguousarray(ky, dtype=np.float32)
+        kz = np.ascontiguousarray(kz, dtype=np.float32)
+        check(lib().bd_set_plume_kernels(self.handle, int(which), _ptr(kx), kx.size, _ptr(ky), ky.size, _ptr(kz),
+                                         kz.size, int(rx), int(ry), int(rz)), "bd_set_plume_kernels")
+
+    # ---- transfers
+    def _shape_of(self, name):
+        return self.plume_shape if name in PLUME_FIELDS else self.shape
+
+    def upload(self, name, arr):
+        """Copy a host array (or scalar for DS_*/CS_* parameters) into a device field."""
+        fid = F[name]
+        if name in MASK_FIELDS:
+            a = np.ascontiguousarray(arr, dtype=np.uint8)
+            dt = BD_U8
+        else:
+            a = np.asarray(arr)
+            if a.dtype == np.float32:
+                a = np.ascontiguousarray(a)
+                dt = BD_F32
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                dt = BD_F64
+        if name.startswith(("DS_", "CS_")):
+            a = a.ravel()
+        check(lib().bd_upload(self.handle, fid, dt, _ptr(a), a.size), f"upload {name}")
+
+    def download(self, name, dtype=np.float64, out=None):
+        fid = F[name]
+        shape = self._shape_of(name)
+        if name in MASK_FIELDS:
+            res = out if out is not None else np.empty(shape, dtype=np.uint8)
+            check(lib().bd_download(self.handle, fid, BD_U8, _ptr(res), res.size), f"download {name}")
+            return res
+        if name == "ABLATION_TIME":
+            dtype = np.float64
+        res = out if out is not None else np.empty(shape, dtype=dtype)
+        assert res.flags.c_contiguous and res.dtype in (np.float32, np.float64)
+        dt = BD_F32 if res.dtype == np.float32 else BD_F64
+        check(lib().bd_download(self.handle, fid, dt, _ptr(res), res.size), f"download {name}")
+        return res
+
+    def fill(self, name, value):
+        check(lib().bd_fill(self.handle, F[name], float(value)), f"fill {name}")
+
+    def slice(self, name, plane, index, stride=1):
+        nx, ny, nz = self.shape
+        na = (nx if plane == 1 else ny)
+        out = np.empty(((na + stride - 1) // stride, (nz + stride - 1) // stride), dtype=np.float32)
+        check(lib().bd_download_slice(self.handle, F[name], int(plane), int(index), int(stride), _ptr(out),
+                                      out.size), f"slice {name}")
+        return out
+
+    # ---- compute
+    def substeps(self, n, is_active=1, flags=0):
+        check(lib().bd_substeps(self.handle, int(n), int(is_active), int(flags)), "bd_substeps")
+
+    def plateau(self):
+        check(lib().bd_plateau(self.handle), "bd_plateau")
+
+    def arrhenius(self, A, EaR, dt):
+        check(lib().bd_arrhenius(self.handle, float(A), float(EaR), float(dt)), "bd_arrhenius")
+
+    def char(self, A, EaR, dt):
+        check(lib().bd_char(self.handle, float(A), float(EaR), float(dt)), "bd_char")
+
+    def plume_substeps(self, n, dt, decay):
+        check(lib().bd_plume_substeps(self.handle, int(n), float(dt), float(decay)), "bd_plume_substeps")
+
+    def mc_run(self, beam: Beam) -> MCResult:
+        r = MCResult()
+        check(lib().bd_mc_run(self.handle, ctypes.byref(beam), ctypes.byref(r)), "bd_mc_run")
+        return r
+
+    def mc_lateral(self, n_depth):
+        out = np.empty(int(n_depth), dtype=np.float64)
+        check(lib().bd_mc_lateral(self.handle, _ptr(out), out.size), "bd_mc_lateral")
+        return out
+
+    def phase_update(self, t_now):
+        n = ctypes.c_int64(0)
+        check(lib().bd_phase_update(self.handle, float(t_now), ctypes.byref(n)), "bd_phase_update")
+        return int(n.value)
+
+    def plume_void(self):
+        check(lib().bd_plume_void(self.handle), "bd_plume_void")
+
+    def plume_inject(self):
+        check(lib().bd_plume_inject(self.handle), "bd_plume_inject")
+
+    def plume_sync(self):
+        check(lib().bd_plume_sync(self.handle), "bd_plume_sync")
+
+    def plume_update_separable(self, dt_epoch):
+        check(lib().bd_plume_update_separable(self.handle, float(dt_epoch)), "bd_plume_update_separable")
+
+    def plume_splat(self, which, impulse, x0, y0, z0, delta_mu):
+        imp = np.ascontiguousarray(impulse, dtype=np.float32)
+        Lx, Ly, Lz = imp.shape
+        check(lib().bd_plume_splat(self.handle, int(which), _ptr(imp), Lx, Ly, Lz, int(x0), int(y0), int(z0),
+                                   float(delta_mu)), "bd_plume_splat")
+
+    def plume_diag(self):
+        out = (ctypes.c_double * 4)()
+        check(lib().bd_plume_diag(self.handle, out), "bd_plume_diag")
+        return tuple(float(v) for v in out)
+
+    def epoch(self, beam: Beam, n_substeps, t_after) -> EpochDiag:
+        d = EpochDiag()
+        check(lib().bd_epoch(self.handle, ctypes.byref(beam), int(n_substeps), float(t_after), ctypes.byref(d)),
+              "bd_epoch")
+        return d
+
+    def reduce(self, name, op="sum", mask=None):
+        ops = {"sum": 0, "max": 1, "min": 2}
+        out = ctypes.c_double(0.0)
+        check(lib().bd_reduce(self.handle, F[name], ops[op], F[mask] if mask else -1, ctypes.byref(out)),
+              f"reduce {name}")
+        return float(out.value)
+
+    def phase_map(self, is_air_gap=None):
+        out = np.empty(self.shape, dtype=np.uint8)
+        gap = None if is_air_gap is None else np.ascontiguousarray(is_air_gap, dtype=np.uint8)
+        check(lib().bd_phase_map(self.handle, None if gap is None else _ptr(gap), _ptr(out), out.size),
+              "bd_phase_map")
+        return out
+
+    def sync(self):
+        check(lib().bd_sync(self.handle), "bd_sync")
+
+
+_cache = {}
+
+
+def scratch_context(shape, spacing) -> Context:
+    """A cached context per (shape, spacing) for the stand-alone module kernels."""
+    key = (tuple(int(v) for v in shape), tuple(float(v) for v in spacing))
+    ctx = _cache.get(key)
+    if ctx is None:
+        if len(_cache) > 8:
+            _cache.pop(next(iter(_cache))).close()
+        ctx = Context(*key)
+        _cache[key] = ctx
+    return ctx

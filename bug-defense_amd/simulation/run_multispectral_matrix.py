@@ -1,0 +1,45 @@
+"""Constants and the phase map of ``simulation/run_multispectral_matrix.py``.
+
+Only the parts of the reference's matrix runner that the engine callers import
+are kept here (:54, :69-78, :168-192): ``H_LAT_WATER``, ``CHAR_STATE`` and
+``compute_phase_map``.  The beetle model builder needs the tissue-optics
+package, which is outside this build.
+"""
+
+import numpy as np
+
+from . import _bdlib as B
+
+T_AMBIENT = 20.0
+T_BOIL = 100.0
+T_ABL_CUTICLE = 350.0
+H_LAT_WATER = 2.26
+H_EFF = 15.0e-6
+
+CHAR_STATE = {
+    'A': 1.0e7,
+    'Ea': 65000.0,
+    'mu_a': 15.0,
+    'mu_s': 20.0,
+    'k': 0.35e-3,
+    'rho_cp': 2.0e-3,
+    't_remove': T_ABL_CUTICLE,
+    'chi_remove': 0.95,
+}
+
+
+def compute_phase_map(T, chi, dry, ablated, H_acc, is_air_gap=None):
+    """run_multispectral_matrix.py:168-192 evaluated on the device.
+
+    0 intact, 1 char (dry and chi >= 0.10), 2 boiling (T >= 99.5 or H_acc > 0),
+    3 void (ablated or air gap).  Inputs are host arrays; for an engine's own
+    state use ``CoupledAblationEngine.phase_map`` (no host round trip).
+    """
+    T = np.asarray(T)
+    ctx = B.scratch_context(T.shape, (1.0, 1.0, 1.0))
+    ctx.upload('T', T)
+    ctx.upload('CHI', chi)
+    ctx.upload('H_ACC', H_acc)
+    ctx.upload('DRY', dry)
+    ctx.upload('ABLATED', ablated)
+    return ctx.phase_map(is_air_gap)

@@ -1,0 +1,229 @@
+/*
+ * bugdef.h -- C ABI of libbugdef.so, the MI355X (gfx950) implementation of the
+ * coupled laser-tissue epoch loop: voxel Monte Carlo photon transport, the
+ * explicit bioheat stencil with the enthalpy plateau, Arrhenius damage and
+ * char kinetics, the boil/char/ablate property update with void conversion,
+ * and void-confined plume diffusion.
+ *
+ * Conventions
+ *  - Every call returns an int status: BD_OK (0) or a negative BD_E* code;
+ *    bd_last_error() returns a message for the calling thread's last failure.
+ *  - Arrays are C-order (Nx, Ny, Nz): flat index (i*Ny + j)*Nz + k, z fastest,
+ *    the layout of the reference's numpy grids.
+ *  - The library owns all device memory; the caller owns host buffers.
+ *    Uploads and downloads are synchronous with respect to the host.
+ *  - Units are the reference engine's (simulation/coupled_ablation.py:219-221):
+ *    mm, 1/mm, W/(mm K), J/(mm^3 K), J/mm^3, W/mm^3, degC, s.
+ *  - One host thread per context; work runs on the context's own HIP stream.
+ */
+#ifndef BUGDEF_H
+#define BUGDEF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BD_OK 0
+#define BD_EINVAL (-1)   /* bad argument: maps to ValueError */
+#define BD_ENODEV (-2)   /* no HIP device: maps to RuntimeError */
+#define BD_EHIP (-3)     /* HIP runtime failure: maps to RuntimeError */
+#define BD_ESTATE (-4)   /* call not valid in the context's current state */
+
+#define BD_F32 0
+#define BD_F64 1
+#define BD_U8 2
+
+/* Field identifiers (device-resident grids of the context). */
+enum bd_field {
+    BD_FIELD_T = 0,          /* temperature [degC] (f32) */
+    BD_FIELD_H_ACC,          /* banked latent enthalpy [J/mm^3] (f32) */
+    BD_FIELD_OMEGA,          /* Arrhenius damage integral (f32) */
+    BD_FIELD_CHI,            /* char fraction (f32) */
+    BD_FIELD_Q,              /* volumetric source [W/mm^3] (f32) */
+    BD_FIELD_K,              /* conductivity (f32) */
+    BD_FIELD_RHO_CP,         /* volumetric heat capacity (f32) */
+    BD_FIELD_H_LAT,          /* latent enthalpy to ablate (f32) */
+    BD_FIELD_DRY,            /* dehydrated mask (u8) */
+    BD_FIELD_ABLATED,        /* void mask (u8) */
+    BD_FIELD_MU_A, BD_FIELD_MU_S, BD_FIELD_G, BD_FIELD_N,   /* optics (f32) */
+    BD_FIELD_WET_MU_A, BD_FIELD_WET_K, BD_FIELD_WET_RHO_CP, /* wet copies (f32) */
+    BD_FIELD_T_PEAK,         /* running max of T at epoch ends (f32) */
+    BD_FIELD_ABLATION_TIME,  /* time of removal, -1 intact (f64) */
+    BD_FIELD_A,              /* absorbed photon weight tally (f32) */
+    BD_FIELD_L,              /* photon path-length tally (f32) */
+    BD_FIELD_PLUME_RAW,      /* plume raw extinction, plume grid (f32) */
+    BD_FIELD_PLUME_EXT,      /* saturated plume extinction, plume grid (f32) */
+    BD_FIELD_VOID,           /* plume void mask, plume grid (u8) */
+    BD_FIELD_TO_REMOVE,      /* voxels removed by the last phase update (u8) */
+    /* dry-state / char-state parameters: upload n == 1 for a scalar,
+     * n == Nx*Ny*Nz for a per-voxel array (coupled_ablation.py:123-125) */
+    BD_FIELD_DS_MU_A, BD_FIELD_DS_MU_S, BD_FIELD_DS_G, BD_FIELD_DS_N,
+    BD_FIELD_DS_K, BD_FIELD_DS_RHO_CP, BD_FIELD_DS_T_REMOVE,
+    BD_FIELD_CS_MU_A, BD_FIELD_CS_MU_S, BD_FIELD_CS_K, BD_FIELD_CS_RHO_CP,
+    BD_FIELD_CS_T_REMOVE,
+    BD_FIELD_COUNT
+};
+
+typedef struct bd_ctx bd_ctx;
+
+typedef struct {
+    int32_t nx, ny, nz;
+    double dx, dy, dz;
+} bd_grid;
+
+/* Engine scalars (coupled_ablation.py:184-311). */
+typedef struct {
+    double dt_sub, h_eff, t_ambient, t_abl, t_void;
+    int32_t has_arrhenius;
+    double arr_A, arr_Ea_over_R;
+    int32_t has_char;
+    double char_A, char_Ea_over_R;
+    int32_t phase_mode;            /* 0: single-stage (:61-74), 1: dry state (:77-180) */
+    int32_t has_cs, has_cs_mu_a, has_cs_mu_s, has_cs_k, has_cs_rho_cp;
+    double cs_chi_remove, alpha_max_stable;
+    double k_void, rho_cp_void;
+    int32_t air_gap_k0, air_gap_k1; /* air_gap_range, or -1 -1 for none */
+    int32_t no_removal;            /* 1: bd_phase_update only writes TO_REMOVE (the
+                                      module kernels); 0: also applies _remove */
+} bd_physics;
+
+/* Plume closure (plume.py:142-268).  The plume grid may equal the thermal
+ * grid (the reference layout) or be a coarser grid covering the same box. */
+typedef struct {
+    int32_t enabled;
+    int32_t model;                 /* 0: diffusion, 1: separable (legacy) */
+    int32_t nx, ny, nz;            /* plume grid */
+    double D_xy, D_z, v_drift, tau_linger;
+    double mu_max, mu_a_max, mu_s_max, delta_mu;
+    int32_t blast_expansion;
+} bd_plume_params;
+
+/* Beam and run controls for one Monte Carlo call (voxel_mc_3d.py:425-445). */
+typedef struct {
+    double power, spot_dia, theta_rad, phi_rad;
+    double x_center, y_center, z_center, n_ambient, r_max;
+    int32_t beam_profile;          /* 0 gaussian, 1 tophat, 2 kim_exp_disk, 3 kim_gauss_disk */
+    int32_t launch_axis;           /* 0: +z from z=0, 1: +x from x=0 */
+    int64_t n_photons, photon_offset, max_events;
+    uint64_t seed;
+    int32_t score_fluence;         /* 1: also tally path length (Phi) */
+    int32_t add_plume;             /* 1: add plume extinction to the optics */
+} bd_beam;
+
+typedef struct {
+    double specular, diffuse, transmitted, lateral, absorbed; /* weight sums */
+    double events;                 /* transport events traced */
+    double kernel_ms;              /* device time of the transport kernel */
+} bd_mc_result;
+
+/* Per-epoch diagnostics (coupled_ablation.py:605-617) plus FP64 ledgers. */
+typedef struct {
+    int64_t n_new;
+    double absorbed_frac, mc_energy_closure;
+    double plume_tau, plume_transmission, plume_peak_mu_a, plume_mean_mu_a;
+    double t_optical_ms, t_thermal_ms, t_phase_ms;
+    double absorbed_J, reflected_J, transmitted_J, lateral_J, conducted_J, ablated_J;
+    double R_spec, R_diff, transmission, lateral_loss;  /* MC fractions (voxel_mc_3d.py:665-670) */
+    double mc_events;
+    double t_substeps_ms;          /* device time of the fused thermal sub-step launches alone */
+    double t_mc_kernel_ms;         /* device time of the transport kernel alone */
+} bd_epoch_diag;
+
+/* Device / library */
+const char *bd_last_error(void);
+int bd_device_count(int *count);
+int bd_version(int *major, int *minor);
+
+/* Context lifetime: replaces MetalAblationContext.__init__
+ * (simulation/metal_ablation_solver.py:290-379) and the engine's array
+ * allocation (coupled_ablation.py:272-307). */
+int bd_create(const bd_grid *grid, bd_ctx **out);
+int bd_destroy(bd_ctx *ctx);
+int bd_set_physics(bd_ctx *ctx, const bd_physics *p);
+int bd_set_plume(bd_ctx *ctx, const bd_plume_params *p);
+/* Separable splat kernels (plume.py:238-268); which = 0 blast, 1 legacy. */
+int bd_set_plume_kernels(bd_ctx *ctx, int which, const float *kx, int nkx, const float *ky, int nky,
+                         const float *kz, int nkz, int rx, int ry, int rz);
+
+/* Host <-> device field copies: replace MetalAblationContext.set_state /
+ * get_state / get_plume (metal_ablation_solver.py:381-455) and the Metal MC
+ * buffer copies (voxel_mc_metal.py:487-506, 535-586). */
+int bd_upload(bd_ctx *ctx, int field, int dtype, const void *host, int64_t n);
+int bd_download(bd_ctx *ctx, int field, int dtype, void *host, int64_t n);
+int bd_fill(bd_ctx *ctx, int field, double value);
+/* Sub-volume copy of one field: an x-z plane at j (plane = 1) or a y-z plane
+ * at i (plane = 0), every `stride`-th point, as f32 (run_highres_knowledge_suite.py:271-306). */
+int bd_download_slice(bd_ctx *ctx, int field, int plane, int index, int stride, float *host, int64_t n);
+
+/* Thermal sub-steps: fused _thermal3d_step_explicit + _apply_latent_heat_plateau
+ * + _accumulate_arrhenius + _accumulate_char (voxel_thermal_3d.py:30-143,
+ * coupled_ablation.py:33-58, 515-541), plume diffusion included when enabled;
+ * replaces MetalAblationContext.advance_substeps (metal_ablation_solver.py:399-437).
+ * flags: bit0 plateau, bit1 arrhenius, bit2 char, bit3 zero Q in ablated
+ * cells, bit4 track T_peak after the last sub-step, bit5 plume diffusion. */
+#define BD_SUB_PLATEAU 1
+#define BD_SUB_ARRHENIUS 2
+#define BD_SUB_CHAR 4
+#define BD_SUB_MASK_Q 8
+#define BD_SUB_PEAK 16
+#define BD_SUB_PLUME 32
+int bd_substeps(bd_ctx *ctx, int n_substeps, int is_active, int flags);
+
+/* Stand-alone module kernels on the context's fields, for callers of the
+ * reference's module functions (same arithmetic as bd_substeps). */
+int bd_plateau(bd_ctx *ctx);                                      /* voxel_thermal_3d.py:115-143 */
+int bd_arrhenius(bd_ctx *ctx, double A, double Ea_over_R, double dt); /* coupled_ablation.py:33-40 */
+int bd_char(bd_ctx *ctx, double A, double Ea_over_R, double dt);      /* coupled_ablation.py:43-58 */
+int bd_plume_substeps(bd_ctx *ctx, int n, double dt, double decay);   /* plume.py:18-80, 333-350 */
+
+/* Monte Carlo transport: replaces run_voxel_mc_metal / _mc3d_trace_photons
+ * (voxel_mc_metal.py:388-586, voxel_mc_3d.py:340-422).  Tallies A and L are
+ * zeroed first; Q is set to A * power / (n_photons * voxel volume). */
+int bd_mc_run(bd_ctx *ctx, const bd_beam *beam, bd_mc_result *out);
+/* Lateral escape per depth index (voxel_mc_3d.py:697), n_depth values. */
+int bd_mc_lateral(bd_ctx *ctx, double *host, int64_t n_depth);
+
+/* Phase update + removal: _update_phase_properties_numba or
+ * _update_single_stage_ablation_numba, then _remove with air-gap perforation
+ * (coupled_ablation.py:61-180, 551-583, 628-654).  t_now is the engine time
+ * written into ablation_time. */
+int bd_phase_update(bd_ctx *ctx, double t_now, int64_t *n_new);
+
+/* Plume bookkeeping (plume.py:292-366, coupled_ablation.py:587-603, 619-626):
+ * void mask from the ablated field (with air-gap breach), injection of the
+ * voxels removed by the last phase update, tanh saturation, legacy update,
+ * centre-column optical depth and peak/mean diagnostics. */
+int bd_plume_void(bd_ctx *ctx);
+int bd_plume_inject(bd_ctx *ctx);
+int bd_plume_sync(bd_ctx *ctx);
+int bd_plume_update_separable(bd_ctx *ctx, double dt_epoch);
+/* _convolve_separable_plume_numba (plume.py:83-139) on the plume raw field with
+ * an explicit float32 impulse box (Lx,Ly,Lz) and output origin (x0,y0,z0);
+ * which = 0 blast taps, 1 legacy taps (bd_set_plume_kernels). */
+int bd_plume_splat(bd_ctx *ctx, int which, const float *impulse, int Lx, int Ly, int Lz, int x0, int y0, int z0,
+                   double delta_mu);
+int bd_plume_diag(bd_ctx *ctx, double out[4]); /* tau, transmission, peak, mean */
+
+/* One whole epoch (coupled_ablation.py:440-617): MC when beam->power > 0,
+ * n_substeps fused sub-steps, T_peak, phase update and removal, plume
+ * injection/sync and diagnostics, with FP64 energy ledgers. */
+int bd_epoch(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_after, bd_epoch_diag *out);
+
+/* Reductions (FP64): op 0 sum, 1 max, 2 min over a field; optional u8 mask
+ * field (-1 for none) restricts the reduction to mask != 0. */
+int bd_reduce(bd_ctx *ctx, int field, int op, int mask_field, double *out);
+
+/* compute_phase_map (run_multispectral_matrix.py:168-192) on the device;
+ * is_air_gap may be NULL.  out is uint8 (Nx,Ny,Nz). */
+int bd_phase_map(bd_ctx *ctx, const uint8_t *is_air_gap, uint8_t *out, int64_t n);
+
+/* Wait for all queued work of the context. */
+int bd_sync(bd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BUGDEF_H */

@@ -494,7 +494,7 @@ class OracleEngine:
                  beam_center=None, stability_factor=0.35, k_void=K_AIR, rho_cp_void=RHO_CP_AIR, arrhenius=None,
                  beam_profile='gaussian', dry_state=None, char_state=None, plume_closure=None, theta_inc_deg=0.0,
                  phi_inc_deg=0.0, initial_dry_mask=None, initial_ablated_mask=None, air_gap_range=None,
-                 wavelength_nm=10600.0, seed=0, mc_threads=None, **_ignored):
+                 wavelength_nm=10600.0, seed=0, mc_threads=None, plume_grid=None, **_ignored):
         self.optical = {k: np.array(optical_props[k], dtype=np.float64, copy=True) for k in ('mu_a', 'mu_s', 'g', 'n')}
         self.Nx, self.Ny, self.Nz = self.optical['mu_a'].shape
         self.Lx, self.Ly, self.Lz = float(Lx), float(Ly), float(Lz)
@@ -571,11 +571,18 @@ class OracleEngine:
         self.air_gap_range = air_gap_range
         self.wavelength_nm = float(wavelength_nm)
         self.plume = None
+        # plume grid: the thermal grid (reference layout) or a coarser grid over the same box,
+        # coupled by cell-centre sampling (thermal -> plume index: (2i+1) P // (2N), and back)
+        self.pshape = tuple(int(v) for v in plume_grid) if plume_grid is not None else shape
+        self._same_plume = self.pshape == shape
+        self._tp = [((2 * np.arange(N) + 1) * P) // (2 * N) for N, P in zip(shape, self.pshape)]
+        self._pt = [((2 * np.arange(P) + 1) * N) // (2 * P) for N, P in zip(shape, self.pshape)]
         if isinstance(plume_closure, dict) and plume_closure.get('enabled', True):
             pc = plume_closure
             D_plume = float(pc.get('D_plume', pc.get('D', 2.0)))
             self.plume = OraclePlume(
-                shape=shape, dx_mm=self.dx, dy_mm=self.dy, dz_mm=self.dz,
+                shape=self.pshape, dx_mm=self.Lx / self.pshape[0], dy_mm=self.Ly / self.pshape[1],
+                dz_mm=self.Lz / self.pshape[2],
                 wavelength_nm=float(pc.get('wavelength_nm', self.wavelength_nm)),
                 tau_linger_s=float(pc.get('tau_linger_s', 0.00875)),
                 sigma_xy_mm=float(pc.get('sigma_xy_mm', pc.get('sigma_mm', 0.060))),
@@ -633,6 +640,23 @@ class OracleEngine:
                 v[:, :, k0:k1] = True
         return v
 
+    def _to_plume(self, a):
+        """thermal-grid mask -> plume grid (value under each plume cell centre)"""
+        return a if self._same_plume else a[np.ix_(*self._pt)]
+
+    def _from_plume(self, a):
+        """plume-grid field -> thermal grid (plume cell under each thermal cell centre)"""
+        return a if self._same_plume else a[np.ix_(*self._tp)]
+
+    def _inject(self, newly):
+        if self._same_plume:
+            self.plume.inject(newly, void_mask=self.compute_void_mask())
+            return
+        ii, jj, kk = np.nonzero(newly)
+        amount = np.float32(self.plume.delta_mu * (np.prod(self.pshape) / newly.size))
+        np.add.at(self.plume.raw_field, (self._tp[0][ii], self._tp[1][jj], self._tp[2][kk]), amount)
+        self.plume.raw_field[~self._to_plume(self.compute_void_mask())] = 0.0
+
     def _remove(self, newly):
         """coupled_ablation.py:628-660."""
         n_new = int(np.sum(newly))
@@ -659,7 +683,7 @@ class OracleEngine:
                                 self.ablated[ix, iy, k0:k1] = True
                                 self.ablation_time[ix, iy, k0:k1] = self.t
             if self.plume is not None:
-                self.plume.inject(newly, void_mask=self.compute_void_mask())
+                self._inject(newly)
         return n_new
 
     def step_epoch(self):
@@ -668,9 +692,9 @@ class OracleEngine:
         if self.power > 0.0:
             if self.plume is not None and np.any(self.plume.mu_ext_plume > 0.0):
                 opt = {k: v.copy() for k, v in self.optical.items()}
-                opt['mu_a'] = self.optical['mu_a'] + self.plume.mu_a_plume
-                opt['mu_s'] = self.optical['mu_s'] + self.plume.mu_s_plume
-                opt['g'][(self.plume.mu_ext_plume > 0.001) & self.ablated] = 0.80
+                opt['mu_a'] = self.optical['mu_a'] + self._from_plume(self.plume.mu_a_plume)
+                opt['mu_s'] = self.optical['mu_s'] + self._from_plume(self.plume.mu_s_plume)
+                opt['g'][(self._from_plume(self.plume.mu_ext_plume) > 0.001) & self.ablated] = 0.80
             else:
                 opt = self.optical
             mc = run_voxel_mc3d(power=self.power, n_photons=self.n_photons, spot_dia=self.spot_dia, Lx=self.Lx,
@@ -700,7 +724,7 @@ class OracleEngine:
         void = self.compute_void_mask() if has_diff else self.ablated
         ab8 = self.ablated.view(np.uint8)
         dr8 = self.dry.view(np.uint8)
-        v8 = np.ascontiguousarray(void, dtype=np.uint8)
+        v8 = np.ascontiguousarray(self._to_plume(void), dtype=np.uint8)
         T, Tn = self.T, self._T_next
         if has_diff:
             P, Pn = self.plume.raw_field, self.plume._tmp_field
@@ -716,8 +740,8 @@ class OracleEngine:
             if self._arrhenius_char is not None:
                 char(Tn, self.chi, dr8, ab8, self._arrhenius_char[0], self._arrhenius_char[1], self.dt_sub)
             if has_diff:
-                plume_diffuse(P, Pn, v8, *N, self.dx, self.dy, self.dz, self.dt_sub, self.plume.D_xy,
-                              self.plume.D_z, self.plume.v_drift, decay)
+                plume_diffuse(P, Pn, v8, *self.pshape, self.plume.dx, self.plume.dy, self.plume.dz, self.dt_sub,
+                              self.plume.D_xy, self.plume.D_z, self.plume.v_drift, decay)
                 P, Pn = Pn, P
             T, Tn = Tn, T
         self.T, self._T_next = T, Tn
@@ -740,17 +764,17 @@ class OracleEngine:
                          self._cs_chi_remove, self.alpha_max_stable, rem.view(np.uint8))
         n_new = self._remove(rem)
         if self.plume is not None:
-            vm = self.compute_void_mask()
+            vm = self._to_plume(self.compute_void_mask())
             if self.plume.model == 'diffusion':
                 self.plume.sync_extinction(vm)
             else:
                 self.plume.update(self.dt_epoch, rem, vm)
-            mx, my = self.Nx // 2, self.Ny // 2
+            mx, my = self._tp[0][self.Nx // 2], self._tp[1][self.Ny // 2]
             mua_p = self.plume.mu_a_plume
-            tau = float(np.sum(mua_p[mx, my, :]) * self.dz)
+            tau = float(np.sum(mua_p[mx, my, :]) * self.plume.dz)
             trans = float(np.exp(-tau))
             peak = float(np.max(mua_p))
-            mean = float(np.mean(mua_p[self.ablated])) if np.any(self.ablated) else 0.0
+            mean = float(np.mean(self._from_plume(mua_p)[self.ablated])) if np.any(self.ablated) else 0.0
         else:
             tau, trans, peak, mean = 0.0, 1.0, 0.0, 0.0
         return {'epoch': self.epoch, 'time_s': self.t, 'n_new': n_new, 'absorbed_frac': absorbed,

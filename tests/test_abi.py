@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports every entry point include/bugdef.h declares (CPU only)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "bugdef.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bd_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from simulation import _bdlib as B
+    lib = B.lib()
+    names = _declared()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    # and the Python binding covers the same set
+    assert set(names) == set(B.EXPORTS)
+
+
+def test_struct_sizes_match_header():
+    """The ctypes mirrors must have the C layout (a mismatch corrupts every call)."""
+    from simulation import _bdlib as B
+    assert ctypes.sizeof(B.Grid) == 3 * 4 + 4 + 3 * 8  # int32 x3, pad, double x3
+    assert ctypes.sizeof(B.MCResult) == 7 * 8
+    assert ctypes.sizeof(B.EpochDiag) == 8 + 22 * 8
+
+
+def test_no_device_raises_loudly():
+    """Without a HIP device the product refuses to run (no silent CPU fallback)."""
+    from simulation import _bdlib as B
+    if B.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    with pytest.raises(RuntimeError):
+        run_voxel_mc3d(n_photons=100, Nx=4, Ny=4, Nz=4)
+    with pytest.raises(RuntimeError):
+        B.Context((4, 4, 4), (0.1, 0.1, 0.1))
+
+
+def test_cpu_engine_keyword_rejected():
+    from simulation.voxel_mc_3d import check_engine
+    with pytest.raises(ValueError):
+        check_engine('cpu')
+    with pytest.raises(ValueError):
+        check_engine('metal')

@@ -1,0 +1,230 @@
+"""GPU engine (phase update, removal, plume, full epochs) against the FP64 oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _f32(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def _phase_inputs(seed, shape):
+    rng = np.random.default_rng(seed)
+    H = _f32(np.where(rng.random(shape) < 0.5, rng.uniform(0, 3.0, shape), 0.0))
+    hl = _f32(np.where(rng.random(shape) < 0.8, 2.26, 0.0))
+    T = _f32(rng.uniform(20.0, 400.0, shape))
+    chi = _f32(rng.uniform(0, 1, shape))
+    chi.flat[:20] = np.float32(0.95)  # exactly at the removal threshold as stored
+    dry = rng.random(shape) < 0.4
+    abl = rng.random(shape) < 0.1
+    opt = {k: _f32(rng.uniform(0.5, 2.0, shape)) for k in ("mu_a", "mu_s", "g", "n")}
+    k = _f32(rng.uniform(0.2e-3, 0.6e-3, shape))
+    rc = _f32(rng.uniform(2e-3, 4e-3, shape))
+    return H, hl, T, chi, dry, abl, opt, k, rc
+
+
+def test_phase_update_masks_bit_exact(oracle):
+    """_update_phase_properties_numba (coupled_ablation.py:77-180): given the same stored fields the
+    device's dry and to_remove masks are bit-exact with the FP64 oracle; properties within FP32."""
+    from simulation.coupled_ablation import _update_phase_properties_numba
+    shape = (23, 19, 31)
+    H, hl, T, chi, dry, abl, opt, k, rc = _phase_inputs(3, shape)
+    ds = dict(mu_a=5.0, mu_s=7.0, g=0.6, n=1.3, k=0.25e-3, rho_cp=1.5e-3, t_remove=350.0)
+    cs = dict(mu_a=50.0, mu_s=10.0, k=0.8e-3, rho_cp=1.0e-3, t_remove=300.0)
+    alpha = 0.35 / (5e-5 * 3 / 0.015 ** 2)
+    wet = (opt["mu_a"].copy(), k.copy(), rc.copy())
+    o = {kk: v.copy() for kk, v in opt.items()}
+    Ho, ko, rco, dro = H.copy(), k.copy(), rc.copy(), dry.astype(np.uint8).copy()
+    rem_o = np.zeros(shape, np.uint8)
+    oracle.phase_update(Ho, hl, T, chi, dro, abl.astype(np.uint8), o["mu_a"], o["mu_s"], o["g"], o["n"], ko, rco,
+                        wet[0], wet[1], wet[2], ds, cs, 0.95, alpha, rem_o)
+    g = {kk: v.copy() for kk, v in opt.items()}
+    Hg, kg, rcg, drg = H.copy(), k.copy(), rc.copy(), dry.copy()
+    rem_g = np.zeros(shape, bool)
+    arr = lambda v: np.array([v])
+    _update_phase_properties_numba(Hg, hl, T, chi, drg, abl, g["mu_a"], g["mu_s"], g["g"], g["n"], kg, rcg, wet[0],
+                                   wet[1], wet[2], arr(5.0), arr(7.0), arr(0.6), arr(1.3), arr(0.25e-3),
+                                   arr(1.5e-3), arr(350.0), True, True, arr(50.0), arr(10.0), arr(0.8e-3),
+                                   arr(1.0e-3), arr(300.0), 0.95, True, True, True, alpha, rem_g)
+    np.testing.assert_array_equal(drg, dro.astype(bool))
+    np.testing.assert_array_equal(rem_g, rem_o.astype(bool))
+    assert rem_o.sum() > 100 and (dro.astype(bool) & ~dry).sum() > 100
+    np.testing.assert_array_equal(Hg == 0.0, Ho == 0.0)
+    for kk in ("mu_a", "mu_s", "g", "n"):
+        np.testing.assert_allclose(g[kk], o[kk], rtol=1e-6)
+    np.testing.assert_allclose(kg, ko, rtol=1e-6)
+    np.testing.assert_allclose(rcg, rco, rtol=1e-6)
+
+
+def test_single_stage_and_phase_map_bit_exact(oracle):
+    from simulation.coupled_ablation import _update_single_stage_ablation_numba
+    from simulation.run_multispectral_matrix import compute_phase_map
+    shape = (17, 13, 29)
+    H, hl, T, chi, dry, abl, opt, k, rc = _phase_inputs(4, shape)
+    rem_o = np.zeros(shape, np.uint8)
+    oracle.single_stage(H, hl, T, 100.0, abl.astype(np.uint8), rem_o)
+    rem_g = np.zeros(shape, bool)
+    _update_single_stage_ablation_numba(H, hl, T, 100.0, abl, rem_g)
+    np.testing.assert_array_equal(rem_g, rem_o.astype(bool))
+    gap = np.zeros(shape, bool)
+    gap[:, :, 5:9] = True
+    T[0, 0, :4] = np.float32(99.5)
+    for air in (None, gap):
+        np.testing.assert_array_equal(compute_phase_map(T, chi, dry, abl, H, air),
+                                      oracle.compute_phase_map(T, chi, dry, abl, H, air))
+
+
+def test_plume_kernels_parity(oracle):
+    """plume.py:18-80 diffusion step and :83-139 separable splat vs the oracle."""
+    from simulation.plume import _convolve_separable_plume_numba, _diffuse_plume_step_numba
+    rng = np.random.default_rng(6)
+    shape = (21, 18, 25)
+    C = rng.uniform(0, 2.0, shape).astype(np.float32)
+    void = rng.random(shape) < 0.7
+    Co, Cg = np.empty_like(C), np.empty_like(C)
+    args = (*shape, 0.03, 0.025, 0.02, 2e-5, 1.0, 2.0, 15.0, float(np.exp(-2e-5 / 0.01)))
+    oracle.plume_diffuse(C, Co, void.astype(np.uint8), *args)
+    _diffuse_plume_step_numba(C, Cg, void, *args)
+    np.testing.assert_allclose(Cg, Co, rtol=2e-6, atol=1e-7)
+    raw_o = rng.uniform(0, 1, shape).astype(np.float32)
+    raw_g = raw_o.copy()
+    imp = (rng.random((4, 3, 5)) < 0.6).astype(np.float32)
+    kx, ky, kz = (rng.random(n).astype(np.float32) for n in (5, 3, 7))
+    oracle.plume_convolve(raw_o, imp, kx, ky, kz, 0.9, -1, 4, 5)
+    _convolve_separable_plume_numba(raw_g, imp, kx, ky, kz, 0.9, -1, 4, 5)
+    np.testing.assert_array_equal(raw_g, raw_o)  # same FP32 accumulation order: bit-exact
+
+
+def test_plume_reference_fixtures_on_gpu():
+    """tests/test_plume.py:110-179,230-266 of the reference on the device plume field."""
+    from simulation.plume import MultispectralPlumeField
+    p = MultispectralPlumeField(shape=(15, 15, 15), model='diffusion', delta_mu_per_voxel=5.0, D_plume=0.5,
+                                v_drift=10.0, blast_expansion=False)
+    newly = np.zeros((15, 15, 15), bool)
+    newly[7, 7, 7] = True
+    p.inject(newly)
+    raw = p.raw_field
+    assert raw[7, 7, 7] == pytest.approx(5.0) and np.sum(raw > 0.0) == 1
+    void = np.zeros((15, 15, 15), bool)
+    void[5:10, 5:10, 5:10] = True
+    for _ in range(10):
+        p.diffuse_substep(void, dt_sub=5e-5)
+    raw = p.raw_field
+    assert raw[7, 7, 7] < 5.0 and raw[7, 7, 8] > 0.0
+    assert np.all(raw[~void] == 0.0) and np.all(raw[0] == 0.0) and np.all(raw[-1] == 0.0)
+    b = MultispectralPlumeField(shape=(25, 25, 25), model='diffusion', delta_mu_per_voxel=1.0, D_plume=2.0,
+                                v_drift=80.0, blast_expansion=True)
+    vb = np.zeros((25, 25, 25), bool)
+    vb[8:17, 8:17, 5:20] = True
+    nb = np.zeros((25, 25, 25), bool)
+    nb[12, 12, 15] = True
+    b.inject(nb, void_mask=vb)
+    rb = b.raw_field
+    assert rb[12, 12, 15] < 0.20 and np.sum((rb > 1e-4) & vb) > 10 and np.all(rb[~vb] == 0.0)
+    a = MultispectralPlumeField(shape=(31, 31, 31), dx_mm=0.02, dy_mm=0.02, dz_mm=0.02, model='diffusion', D_xy=1.0,
+                                D_z=4.0, v_drift=0.0, tau_linger_s=float('inf'), blast_expansion=False)
+    nc = np.zeros((31, 31, 31), bool)
+    nc[15, 15, 15] = True
+    a.inject(nc)
+    full = np.ones((31, 31, 31), bool)
+    for _ in range(20):
+        a.diffuse_substep(full, dt_sub=2e-5)
+    ra = a.raw_field
+    assert ra[17, 15, 15] == pytest.approx(ra[15, 17, 15], rel=1e-4)
+    assert ra[15, 15, 17] > 2.0 * ra[17, 15, 15]
+
+
+def _small_engine_kwargs(**over):
+    N, Nz, voxel = 11, 20, 0.05
+    shape = (N, N, Nz)
+    kw = dict(optical_props={'mu_a': np.full(shape, 10.0), 'mu_s': np.zeros(shape), 'g': np.zeros(shape),
+                             'n': np.ones(shape)},
+              k_grid=np.full(shape, 0.5e-3), rho_cp_grid=np.full(shape, 4.0e-3), h_lat_grid=np.full(shape, 2.0),
+              Lx=N * voxel, Ly=N * voxel, Lz=Nz * voxel, power=5.0, spot_dia=0.3, t_abl=100.0, t_ambient=25.0,
+              dt_epoch=0.002, dt_sub=5e-5, n_photons=10000, seed=5)
+    kw.update(over)
+    return kw
+
+
+def test_ablation_extensions_reference_fixtures():
+    """tests/test_ablation_extensions.py:72-138 of the reference on the device engine."""
+    from simulation.coupled_ablation import CoupledAblationEngine
+    eng = CoupledAblationEngine(**_small_engine_kwargs())
+    assert eng.char_state is None and np.all(eng.chi == 0.0)
+    eng.step_epoch()
+    assert np.all(eng.chi == 0.0)
+    dry_state = dict(mu_a=5.0, mu_s=0.0, g=0.0, n=1.0, k=0.2e-3, rho_cp=2.0e-3, t_remove=600.0)
+    char_state = dict(A=1e8, Ea=60000.0, mu_a=50.0, mu_s=10.0, k=0.8e-3, rho_cp=1.0e-3, t_remove=600.0)
+    eng = CoupledAblationEngine(**_small_engine_kwargs(dry_state=dry_state, char_state=char_state, power=15.0))
+    for _ in range(25):
+        eng.step_epoch()
+    dm = eng.dry & ~eng.ablated
+    assert np.any(dm)
+    np.testing.assert_allclose(eng.optical['mu_a'][dm], 5.0 + eng.chi[dm] * 45.0, rtol=1e-4)
+    eng = CoupledAblationEngine(**_small_engine_kwargs(plume_closure=dict(tau_linger_s=0.010, delta_mu_per_voxel=5.0,
+                                                                          sigma_mm=0.05), power=20.0))
+    hit = False
+    for _ in range(30):
+        d = eng.step_epoch()
+        if d['n_new'] > 0:
+            hit = True
+            assert d['plume_tau'] > 0.0 and d['plume_transmission'] < 1.0 and d['plume_peak_mu_a'] > 0.0
+            assert d['plume_transmission'] == pytest.approx(np.exp(-d['plume_tau']))
+            break
+    assert hit
+    for th in (0.0, 45.0):
+        e = CoupledAblationEngine(**_small_engine_kwargs(theta_inc_deg=th))
+        assert e.step_epoch()['absorbed_frac'] > 0.0
+
+
+def test_engine_against_oracle_engine(oracle):
+    """Coupled epochs with dry state, char, air gap and diffusion plume: the device engine and
+    the FP64 oracle engine trace the same photon streams; compare the state after 6 epochs."""
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.run_multispectral_matrix import CHAR_STATE
+    shape = (32, 32, 40)
+    rng = np.random.default_rng(0)
+    mu_a = np.zeros(shape)
+    mu_s = np.zeros(shape)
+    g = np.zeros(shape)
+    n = np.ones(shape)
+    k = np.full(shape, 0.026e-3)
+    rc = np.full(shape, 1.206e-6)
+    hl = np.full(shape, 1e12)
+    dry = np.zeros(shape, bool)
+    abl = np.zeros(shape, bool)
+    abl[:, :, :4] = True
+    for (a, b), (ma, ms, gg, nn, kk, rr, hh, dd) in (
+            ((4, 7), (8.0, 20.0, 0.85, 1.5, 0.3e-3, 2.4e-3, 0.0, True)),
+            ((10, 40), (3.0, 15.0, 0.9, 1.37, 0.5e-3, 3.6e-3, 2.26, False))):
+        mu_a[:, :, a:b], mu_s[:, :, a:b], g[:, :, a:b], n[:, :, a:b] = ma, ms, gg, nn
+        k[:, :, a:b], rc[:, :, a:b], hl[:, :, a:b], dry[:, :, a:b] = kk, rr, hh, dd
+    opt = {'mu_a': mu_a, 'mu_s': mu_s, 'g': g, 'n': n}
+    ds = {'mu_a': mu_a.copy(), 'mu_s': mu_s.copy(), 'g': g.copy(), 'n': n.copy(), 'k': k.copy(),
+          'rho_cp': rc.copy(), 't_remove': 350.0}
+    common = dict(optical_props=opt, k_grid=k, rho_cp_grid=rc, h_lat_grid=hl, Lx=0.48, Ly=0.48, Lz=0.6,
+                  power=8.0, spot_dia=0.2, t_abl=100.0, t_ambient=20.0, dt_epoch=0.001, dt_sub=5e-5,
+                  n_photons=40000, h_eff=15e-6, theta_inc_deg=22.0, char_state=CHAR_STATE,
+                  plume_closure={'model': 'diffusion', 'D_xy': 1.0, 'D_z': 2.0, 'v_drift': 0.0,
+                                 'tau_linger_s': float('inf'), 'blast_expansion': False},
+                  initial_dry_mask=dry, initial_ablated_mask=abl, air_gap_range=(7, 10), wavelength_nm=532, seed=42)
+    eng = CoupledAblationEngine(dry_state={kk: (v.copy() if isinstance(v, np.ndarray) else v)
+                                           for kk, v in ds.items()}, **common)
+    ref = oracle.OracleEngine(dry_state={kk: (v.copy() if isinstance(v, np.ndarray) else v)
+                                         for kk, v in ds.items()}, mc_threads=8, **common)
+    dg, dr = [], []
+    for _ in range(6):
+        dg.append(eng.step_epoch())
+        dr.append(ref.step_epoch())
+    for a, b in zip(dg, dr):
+        assert abs(a['absorbed_frac'] - b['absorbed_frac']) < 0.01
+        assert abs(a['mc_energy_closure'] - 1.0) < 0.01
+    Tg, Tr = np.asarray(eng.T), ref.T
+    rise = float(Tr.max() - 20.0)
+    assert rise > 50.0
+    assert abs(float(Tg.max()) - float(Tr.max())) < 0.05 * rise
+    assert abs(float(Tg.mean()) - float(Tr.mean())) < 0.02 * float(Tr.mean() - 20.0) + 1e-3
+    # removed-volume and char agree statistically
+    assert abs(int(eng.ablated.sum()) - int(ref.ablated.sum())) <= 0.2 * max(10, int(ref.ablated.sum() - abl.sum()))
+    assert eng.T_peak.max() >= float(Tg.max()) - 1e-4

@@ -1,0 +1,145 @@
+"""GPU transport kernel against the FP64 oracle and the closed forms (through the C-ABI).
+
+The device and the oracle trace the same Philox streams (same seed, same photon
+ids, same draw order), FP32 on the device and FP64 on the CPU, so their
+results are strongly correlated; tolerances are still statistical.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAYERS3 = None
+
+
+def _three_layer():
+    Nx, Ny, Nz = 30, 30, 40
+    mu_a = np.zeros((Nx, Ny, Nz))
+    mu_s = np.zeros_like(mu_a)
+    g = np.zeros_like(mu_a)
+    n = np.zeros_like(mu_a)
+    for (a, b), (ma, ms, gg, nn) in (((0, 10), (0.60, 16.0, 0.85, 1.52)), ((10, 24), (0.05, 4.0, 0.80, 1.34)),
+                                     ((24, 40), (0.70, 18.0, 0.88, 1.42))):
+        mu_a[:, :, a:b], mu_s[:, :, a:b], g[:, :, a:b], n[:, :, a:b] = ma, ms, gg, nn
+    return {'mu_a': mu_a, 'mu_s': mu_s, 'g': g, 'n': n}
+
+
+def test_homogeneous_cube_closure_reference_fixture():
+    """tests/test_gpu_mc.py:14-38 of the reference."""
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    shape = (20, 20, 25)
+    props = {'mu_a': np.full(shape, 0.2), 'mu_s': np.full(shape, 10.0), 'g': np.full(shape, 0.8),
+             'n': np.full(shape, 1.4)}
+    res = run_voxel_mc3d(power=5.0, n_photons=50_000, Lx=1.0, Ly=1.0, Lz=1.0, Nx=20, Ny=20, Nz=25,
+                         custom_3d_props=props, spot_dia=0.20, seed=1234)
+    assert abs(res['energy_closure'] - 1.0) < 0.01
+    s = res['spectro_raw']
+    assert s['Absorption'] > 0.0 and s['R_spec'] > 0.0 and s['R_diff'] > 0.0
+    assert res['engine'] == 'gpu'
+
+
+def test_multilayer_oblique_parity_with_oracle(oracle):
+    """tests/test_gpu_mc.py:40-126 of the reference (theta 30, phi 45, 100k photons),
+    against the oracle, with tolerances tightened from 0.015 to 0.005."""
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    props = _three_layer()
+    kw = dict(power=10.0, n_photons=100_000, Lx=2.0, Ly=2.0, Lz=1.0, Nx=30, Ny=30, Nz=40, custom_3d_props=props,
+              theta_inc_deg=30.0, phi_inc_deg=45.0, spot_dia=0.40, beam_center=(1.0, 1.0), seed=777)
+    g = run_voxel_mc3d(**kw)
+    c = oracle.run_voxel_mc3d(**kw, n_threads=8)
+    th = np.radians(30.0)
+    n1, n2 = 1.0, 1.52
+    ci = np.cos(th)
+    ct = np.sqrt(1.0 - ((n1 / n2) * np.sin(th)) ** 2)
+    R = 0.5 * (((n1 * ci - n2 * ct) / (n1 * ci + n2 * ct)) ** 2 + ((n1 * ct - n2 * ci) / (n1 * ct + n2 * ci)) ** 2)
+    assert abs(g['spectro_raw']['R_spec'] - R) < 1e-6
+    assert abs(g['energy_closure'] - 1.0) < 0.01
+    for key in ('Absorption', 'R_diff', 'Transmission'):
+        assert abs(g['spectro_raw'][key] - c['spectro_raw'][key]) < 0.005, key
+    corr = np.corrcoef(g['A_3d'].sum(axis=(0, 1)), c['A_3d'].sum(axis=(0, 1)))[0, 1]
+    assert corr > 0.999
+
+
+def test_per_voxel_three_sigma_parity(oracle):
+    """Per-voxel absorbed weight: GPU vs oracle at equal photon counts.  With 16 independent
+    batches per side the per-voxel standard error is estimated from the batch scatter; the
+    fraction of well-sampled voxels beyond 3 sigma must stay at the Gaussian rate (< 1 %),
+    and R, T, A agree within 0.5 % absolute."""
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    props = _three_layer()
+    kw = dict(power=1.0, n_photons=25_000, Lx=2.0, Ly=2.0, Lz=1.0, Nx=30, Ny=30, Nz=40, custom_3d_props=props,
+              theta_inc_deg=22.0, spot_dia=0.5)
+    G, C, sg, sc = [], [], [], []
+    for b in range(16):
+        r = run_voxel_mc3d(seed=100 + b, **kw)
+        G.append(r['A_3d'])
+        sg.append(r['spectro_raw'])
+        q = oracle.run_voxel_mc3d(seed=900 + b, n_threads=8, **kw)  # independent streams
+        C.append(q['A_3d'])
+        sc.append(q['spectro_raw'])
+    G, C = np.array(G), np.array(C)
+    mg, mc = G.mean(0), C.mean(0)
+    se = np.sqrt(G.var(0, ddof=1) / 16 + C.var(0, ddof=1) / 16)
+    ok = (mg > 0) & (mc > 0) & (se > 0) & (np.minimum(mg, mc) > 20 * se / 4)
+    z = np.abs(mg - mc)[ok] / se[ok]
+    assert ok.sum() > 1000
+    assert np.mean(z > 3.0) < 0.01, f"{np.mean(z > 3.0):.4f} of {ok.sum()} voxels beyond 3 sigma"
+    for key in ('Absorption', 'R_diff', 'Transmission', 'R_spec'):
+        a = np.mean([s[key] for s in sg])
+        b = np.mean([s[key] for s in sc])
+        assert abs(a - b) < 0.005, key
+
+
+def test_fresnel_slab_closed_form_gpu():
+    """V1c/V1d/V1e (verification_benchmarks.py:102-128) on the device."""
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    n, mu_a, d = 1.5, 1.0, 0.5
+    R = ((n - 1.0) / (n + 1.0)) ** 2
+    e1, e2 = np.exp(-mu_a * d), np.exp(-2.0 * mu_a * d)
+    T_ref = (1 - R) ** 2 * e1 / (1 - R ** 2 * e2)
+    Rd_ref = R * (1 - R) ** 2 * e2 / (1 - R ** 2 * e2)
+    N = 400000
+    r = run_voxel_mc3d(n_photons=N, spot_dia=0.01, Lx=2.0, Ly=2.0, Lz=0.55, Nx=4, Ny=4, Nz=11, seed=3,
+                       layers_config=[{'z_end': 0.5, 'mu_a': mu_a, 'mu_s': 0.0, 'g': 0.0, 'n': n},
+                                      {'z_end': 0.55, 'mu_a': 0.0, 'mu_s': 0.0, 'g': 0.0, 'n': 1.0}])
+    s = r['spectro_raw']
+    assert abs(s['Transmission'] - T_ref) < 4 * np.sqrt(T_ref * (1 - T_ref) / N)
+    assert abs(s['R_diff'] - Rd_ref) < 4 * np.sqrt(Rd_ref * (1 - Rd_ref) / N)
+    assert abs(s['R_spec'] - R) < 1e-7
+
+
+def test_semi_infinite_h_function_gpu():
+    """V2 (verification_benchmarks.py:87-99): Rd = 0.41495 (Chandrasekhar H-function)."""
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    vals = [run_voxel_mc3d(n_photons=100000, spot_dia=0.01, Lx=20.0, Ly=20.0, Lz=4.0, Nx=20, Ny=20, Nz=8, seed=s,
+                           layers_config=[{'z_end': 4.0, 'mu_a': 1.0, 'mu_s': 9.0, 'g': 0.0, 'n': 1.0}]
+                           )['spectro_raw']['R_diff'] for s in range(6)]
+    m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(6)
+    assert abs(m - 0.414947479) < 4 * se + 2e-4
+
+
+@pytest.mark.parametrize("profile", ["gaussian", "tophat", "kim_exp_disk"])
+def test_beam_profiles_gpu_vs_oracle(oracle, profile):
+    """Launch footprint (voxel_mc_3d.py:101-161): surface-absorber moments vs the oracle."""
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    kw = dict(n_photons=100000, spot_dia=0.8, Lx=2.0, Ly=2.0, Lz=0.5, Nx=100, Ny=100, Nz=1, beam_profile=profile,
+              theta_inc_deg=20.0, layers_config=[{'z_end': 0.5, 'mu_a': 1e4, 'mu_s': 0.0, 'g': 0.0, 'n': 1.0}])
+    g = run_voxel_mc3d(seed=5, **kw)['A_3d'][:, :, 0]
+    c = oracle.run_voxel_mc3d(seed=5, n_threads=8, **kw)['A_3d'][:, :, 0]
+    x = (np.arange(100) + 0.5) * 0.02 - 1.0
+    for w in (x[:, None] ** 2 + 0 * x[None, :], 0 * x[:, None] + x[None, :] ** 2):
+        assert (g * w).sum() / g.sum() == pytest.approx((c * w).sum() / c.sum(), rel=0.01)
+
+
+def test_launch_axis_x_and_r_max_gpu(oracle):
+    """launch_axis='x' and the r_max cut (voxel_mc_3d.py:157-177, 209-217, 309-317) vs the oracle."""
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    lay = [{'z_end': 0.3, 'mu_a': 0.5, 'mu_s': 10.0, 'g': 0.8, 'n': 1.4},
+           {'z_end': 1.0, 'mu_a': 1.0, 'mu_s': 5.0, 'g': 0.9, 'n': 1.33}]
+    kw = dict(n_photons=100000, spot_dia=0.3, Lx=1.0, Ly=4.0, Lz=4.0, Nx=25, Ny=20, Nz=20, layers_config=lay,
+              launch_axis='x', r_max=0.4)
+    g = run_voxel_mc3d(seed=11, **kw)
+    c = oracle.run_voxel_mc3d(seed=11, n_threads=8, **kw)
+    for key in ('R_diff', 'Absorption', 'Transmission', 'Lateral_loss'):
+        assert abs(g['spectro_raw'][key] - c['spectro_raw'][key]) < 0.005, key
+    assert g['lateral_escape_z'].sum() == pytest.approx(g['spectro_raw']['Lateral_loss'], rel=1e-5)
