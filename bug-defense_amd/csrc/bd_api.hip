@@ -438,16 +438,47 @@ int bd_download_slice(bd_ctx *h, int field, int plane, int index, int stride, fl
     BD_CHECK(n == expect, "bd_download_slice: output size mismatch");
     Ref r;
     BD_TRY(resolve(c, field, &r));
-    BD_CHECK(r.kind != KPARAM && r.n == c->nvox, "bd_download_slice: field is not a thermal-grid field");
-    if (!*r.slot) {
+    const bool plume_field = field == BD_FIELD_PLUME_RAW || field == BD_FIELD_PLUME_EXT;
+    BD_CHECK(r.kind != KPARAM && (r.n == c->nvox || plume_field) && field != BD_FIELD_VOID,
+             "bd_download_slice: field is not a thermal-grid field");
+    if (r.n == 0 || !*r.slot) {
         set_error("bd_download_slice: field has never been written");
         return BD_ESTATE;
     }
-    int kind = r.kind == KF32 ? K_F32 : r.kind == KF64 ? K_F64 : r.kind == KU8 ? K_U8
-             : r.kind == KFLAG ? (r.bit == F_ABL ? K_FLAG_ABL : K_FLAG_DRY) : (r.which ? K_AL_L : K_AL_A);
     BD_TRY(ensure_staging(c, (size_t)n * 4));
-    BD_TRY(launch_slice(c, *r.slot, kind, plane, index, stride, (float *)c->staging, n));
+    if (plume_field) {
+        // plume cells under the thermal plane's cell centres (identity for equal grids)
+        BD_TRY(launch_slice_plume(c, (const float *)*r.slot, plane, index, stride, (float *)c->staging, n));
+    } else {
+        int kind = r.kind == KF32 ? K_F32 : r.kind == KF64 ? K_F64 : r.kind == KU8 ? K_U8
+                 : r.kind == KFLAG ? (r.bit == F_ABL ? K_FLAG_ABL : K_FLAG_DRY) : (r.which ? K_AL_L : K_AL_A);
+        BD_TRY(launch_slice(c, *r.slot, kind, plane, index, stride, (float *)c->staging, n));
+    }
     BD_HIP(hipMemcpyAsync(host, c->staging, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    return BD_OK;
+}
+
+int bd_phase_slice(bd_ctx *h, int plane, int index, int stride, const uint8_t *gap_z, uint8_t *out, int64_t n)
+{
+    BD_CHECK(h && out, "bd_phase_slice: null argument");
+    Ctx *c = &h->c;
+    BD_CHECK(plane == 0 || plane == 1, "bd_phase_slice: plane must be 0 (y-z) or 1 (x-z)");
+    BD_CHECK(stride >= 1, "bd_phase_slice: stride must be >= 1");
+    const int lim = plane == 1 ? c->g.ny : c->g.nx;
+    BD_CHECK(index >= 0 && index < lim, "bd_phase_slice: index outside the grid");
+    const int64_t na = plane == 1 ? (c->g.nx + stride - 1) / stride : (c->g.ny + stride - 1) / stride;
+    BD_CHECK(n == na * ((c->g.nz + stride - 1) / stride), "bd_phase_slice: output size mismatch");
+    BD_TRY(ensure_state(c));
+    BD_TRY(ensure_staging(c, (size_t)n + (size_t)c->g.nz + 64));
+    uint8_t *dev_out = (uint8_t *)c->staging;
+    uint8_t *dev_gap = nullptr;
+    if (gap_z) {
+        dev_gap = dev_out + n;
+        BD_HIP(hipMemcpyAsync(dev_gap, gap_z, (size_t)c->g.nz, hipMemcpyHostToDevice, c->stream));
+    }
+    BD_TRY(launch_phase_slice(c, dev_gap, plane, index, stride, dev_out, n));
+    BD_HIP(hipMemcpyAsync(out, dev_out, (size_t)n, hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     return BD_OK;
 }

@@ -123,6 +123,8 @@ int launch_reduce(Ctx *c, const void *src, int kind, int64_t n, int op, const ui
 int launch_fill(Ctx *c, void *dst, int kind, int64_t n, double v);
 int launch_phase_map(Ctx *c, const uint8_t *air_gap_dev, uint8_t *out_dev);
 int launch_slice(Ctx *c, const void *src, int kind, int plane, int index, int stride, float *out, int64_t n);
+int launch_slice_plume(Ctx *c, const float *src, int plane, int index, int stride, float *out, int64_t n);
+int launch_phase_slice(Ctx *c, const uint8_t *gap_z_dev, int plane, int index, int stride, uint8_t *out, int64_t n);
 
 // element kinds of device buffers
 constexpr int K_F32 = 0, K_F64 = 1, K_U8 = 2, K_FLAG_ABL = 3, K_FLAG_DRY = 4, K_AL_A = 5, K_AL_L = 6;

@@ -247,6 +247,59 @@ __global__ void k_phase_map(const float *__restrict__ T, const float *__restrict
     }
 }
 
+// a plume-grid field sampled at the centres of one thermal-grid plane
+__global__ void k_slice_plume(const float *__restrict__ src, int Nx, int Ny, int Nz, int Px, int Py, int Pz,
+                              int plane, int index, int stride, float *__restrict__ out)
+{
+    const int nzs = (Nz + stride - 1) / stride;
+    const int na = plane == 1 ? (Nx + stride - 1) / stride : (Ny + stride - 1) / stride;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na * nzs; t += gridDim.x * blockDim.x) {
+        const int a = (t / nzs) * stride, kk = (t % nzs) * stride;
+        const int i = plane == 1 ? a : index, j = plane == 1 ? index : a;
+        const int I = (int)(((int64_t)(2 * i + 1) * Px) / (2 * (int64_t)Nx));
+        const int J = (int)(((int64_t)(2 * j + 1) * Py) / (2 * (int64_t)Ny));
+        const int K = (int)(((int64_t)(2 * kk + 1) * Pz) / (2 * (int64_t)Nz));
+        out[t] = src[((int64_t)I * Py + J) * Pz + K];
+    }
+}
+
+int launch_slice_plume(Ctx *c, const float *src, int plane, int index, int stride, float *out, int64_t n)
+{
+    hipLaunchKernelGGL(k_slice_plume, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, src, c->g.nx, c->g.ny,
+                       c->g.nz, c->plume.nx, c->plume.ny, c->plume.nz, plane, index, stride, out);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// run_multispectral_matrix.py:168-192 on one plane; air gap as a per-z profile
+__global__ void k_phase_slice(const float *__restrict__ T, const float *__restrict__ chi,
+                              const float *__restrict__ H, const uint8_t *__restrict__ f,
+                              const uint8_t *__restrict__ gap_z, int Nx, int Ny, int Nz, int plane, int index,
+                              int stride, uint8_t *__restrict__ out)
+{
+    const int nzs = (Nz + stride - 1) / stride;
+    const int na = plane == 1 ? (Nx + stride - 1) / stride : (Ny + stride - 1) / stride;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na * nzs; t += gridDim.x * blockDim.x) {
+        const int a = (t / nzs) * stride, kk = (t % nzs) * stride;
+        const int64_t id = plane == 1 ? ((int64_t)a * Ny + index) * Nz + kk : ((int64_t)index * Ny + a) * Nz + kk;
+        const bool g = gap_z && gap_z[kk];
+        const uint8_t fl = f[id];
+        uint8_t ph = 0;
+        if (((T[id] >= 99.5f) || (H[id] > 0.0f)) && !g) ph = 2;
+        if ((fl & F_DRY) && ((double)chi[id] >= 0.10) && !g) ph = 1;
+        if ((fl & F_ABL) || g) ph = 3;
+        out[t] = ph;
+    }
+}
+
+int launch_phase_slice(Ctx *c, const uint8_t *gap_z_dev, int plane, int index, int stride, uint8_t *out, int64_t n)
+{
+    hipLaunchKernelGGL(k_phase_slice, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->T, c->chi, c->H,
+                       c->flags, gap_z_dev, c->g.nx, c->g.ny, c->g.nz, plane, index, stride, out);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
 int launch_phase_map(Ctx *c, const uint8_t *gap, uint8_t *out)
 {
     const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);

@@ -128,6 +128,8 @@ _SIGS = {
     "bd_reduce": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                   ctypes.c_int),
     "bd_phase_map": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_phase_slice": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                        ctypes.c_int64], ctypes.c_int),
     "bd_sync": ([ctypes.c_void_p], ctypes.c_int),
 }
 EXPORTS = tuple(_SIGS)
@@ -344,6 +346,16 @@ class Context:
         gap = None if is_air_gap is None else np.ascontiguousarray(is_air_gap, dtype=np.uint8)
         check(lib().bd_phase_map(self.handle, None if gap is None else _ptr(gap), _ptr(out), out.size),
               "bd_phase_map")
+        return out
+
+    def phase_slice(self, plane, index, gap_z=None, stride=1):
+        """compute_phase_map on one plane, air gap as a per-z profile (uint8 [Nz])."""
+        nx, ny, nz = self.shape
+        na = (nx if plane == 1 else ny)
+        out = np.empty(((na + stride - 1) // stride, (nz + stride - 1) // stride), dtype=np.uint8)
+        gap = None if gap_z is None else np.ascontiguousarray(gap_z, dtype=np.uint8)
+        check(lib().bd_phase_slice(self.handle, int(plane), int(index), int(stride),
+                                   None if gap is None else _ptr(gap), _ptr(out), out.size), "bd_phase_slice")
         return out
 
     def sync(self):

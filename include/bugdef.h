@@ -220,6 +220,12 @@ int bd_reduce(bd_ctx *ctx, int field, int op, int mask_field, double *out);
  * is_air_gap may be NULL.  out is uint8 (Nx,Ny,Nz). */
 int bd_phase_map(bd_ctx *ctx, const uint8_t *is_air_gap, uint8_t *out, int64_t n);
 
+/* compute_phase_map on one plane (plane/index/stride as bd_download_slice),
+ * with the air gap given as a per-z profile gap_z[Nz] (may be NULL); this is
+ * what the per-epoch recorders need (run_highres_knowledge_suite.py:271-283)
+ * without moving the 3-D grids.  out is uint8, n values. */
+int bd_phase_slice(bd_ctx *ctx, int plane, int index, int stride, const uint8_t *gap_z, uint8_t *out, int64_t n);
+
 /* Wait for all queued work of the context. */
 int bd_sync(bd_ctx *ctx);
 
