@@ -161,6 +161,9 @@ int bd_destroy(bd_ctx *h)
         if (f.ptr) (void)hipFree(f.ptr);
     for (auto &f : c->cs)
         if (f.ptr) (void)hipFree(f.ptr);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->ep) (void)hipFree(c->ep);
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -122,3 +122,179 @@ extern "C" int bd_epoch(bd_ctx *h, const bd_beam *beam, int n_substeps, double t
     out->t_substeps_ms = ms;
     return BD_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Graph-captured epoch.  The same work as bd_epoch, enqueued without any host
+// wait and captured once per (power on/off, beam, n_substeps); every per-epoch
+// scalar is read from the device EpochParams block, which the host refreshes
+// with one 32-byte copy before each replay.  Diagnostics land in the reduction
+// scratch and come back with one copy at the end.  Blast (bounding-box)
+// injection needs a host round trip and is not graph-capturable.
+namespace bd {
+void plume_diag_from(const double *h, double out[4]);
+
+static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
+{
+    const bd_physics &p = c->phys;
+    const bool plume = c->plume.enabled != 0;
+    const bool diff_plume = plume && c->plume.model == 0;
+    if (beam->power > 0.0) {
+        BD_TRY(launch_mc_enqueue(c, beam, c->ep, false));
+        BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
+    } else {
+        BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
+        if (c->AL) BD_TRY(launch_fill(c, c->AL, K_F32, 2 * c->nvox, 0.0));
+        BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
+        BD_TRY(launch_fill(c, c->red + 40, K_F64, 1, 0.0));
+    }
+    if (diff_plume) BD_TRY(launch_plume_void(c));
+    int flags = BD_SUB_PLATEAU | BD_SUB_MASK_Q | BD_SUB_PEAK;
+    if (p.has_arrhenius) flags |= BD_SUB_ARRHENIUS;
+    if (p.has_char) flags |= BD_SUB_CHAR;
+    BD_TRY(launch_fill(c, c->red + 8, K_F64, 1, 0.0));
+    BD_TRY(launch_substeps(c, n_substeps, 1, flags, true));
+    if (diff_plume) {
+        const double tl = c->plume.tau_linger;
+        const double decay = (tl > 0.0 && std::isfinite(tl)) ? std::exp(-p.dt_sub / tl) : 1.0;
+        BD_TRY(launch_plume_substeps(c, n_substeps, p.dt_sub, decay));
+    }
+    BD_TRY(launch_phase_enqueue(c, 0.0, c->ep));
+    if (plume) {
+        BD_TRY(launch_plume_void(c));
+        BD_TRY(launch_plume_inject(c));  // point injection: a no-op when nothing was removed
+        BD_TRY(launch_plume_sync(c));
+        BD_TRY(launch_plume_diag_enqueue(c));
+    }
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red, 48 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipMemcpyAsync((char *)c->host_pinned + 48 * sizeof(double), c->counter, 2 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->stream));
+    return BD_OK;
+}
+
+static bool same_beam(const bd_beam &a, const bd_beam &b)
+{
+    return a.power == b.power && a.spot_dia == b.spot_dia && a.theta_rad == b.theta_rad && a.phi_rad == b.phi_rad &&
+           a.x_center == b.x_center && a.y_center == b.y_center && a.z_center == b.z_center &&
+           a.n_ambient == b.n_ambient && a.r_max == b.r_max && a.beam_profile == b.beam_profile &&
+           a.launch_axis == b.launch_axis && a.n_photons == b.n_photons && a.photon_offset == b.photon_offset &&
+           a.max_events == b.max_events && a.score_fluence == b.score_fluence && a.add_plume == b.add_plume;
+}
+
+}  // namespace bd
+
+extern "C" int bd_epoch_graph(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_before, int64_t epoch,
+                              uint64_t seed_base, bd_epoch_diag *out)
+{
+    BD_CHECK(h && beam && out, "bd_epoch_graph: null argument");
+    BD_CHECK(n_substeps >= 1, "bd_epoch_graph: n_substeps must be >= 1");
+    Ctx *c = &h->c;
+    if (!c->has_phys) {
+        set_error("bd_epoch_graph: bd_set_physics has not been called");
+        return BD_ESTATE;
+    }
+    if (c->plume.enabled && (c->plume.model != 0 || c->plume.blast_expansion)) {
+        set_error("bd_epoch_graph: the blast / separable plume needs host round trips; use bd_epoch");
+        return BD_ESTATE;
+    }
+    if (n_substeps % 2) {
+        set_error("bd_epoch_graph: an odd sub-step count leaves the ping-pong buffers swapped; use bd_epoch");
+        return BD_ESTATE;
+    }
+    BD_TRY(ensure_state(c));
+    BD_TRY(ensure(c, (void **)&c->ep, sizeof(EpochParams)));
+    BD_TRY(ensure(c, (void **)&c->Tpeak, (size_t)c->nvox * 4));
+    BD_TRY(ensure(c, (void **)&c->to_remove, (size_t)c->nvox));
+    if (c->phys.has_arrhenius) BD_TRY(ensure(c, (void **)&c->Om, (size_t)c->nvox * 4));
+    const bool recapture = !c->graph_exec || c->graph_nsub != n_substeps || !same_beam(c->graph_beam, *beam) ||
+                           c->graph_T != c->T || c->graph_P != c->P;
+    if (recapture) {
+        // allocate every lazily created buffer and tune the tiles before capture
+        BD_TRY(launch_substeps(c, 0, 1, BD_SUB_PLATEAU, true));
+        if (beam->power > 0.0) BD_TRY(launch_pack_optics(c, beam->add_plume != 0));
+        if (c->graph_exec) {
+            (void)hipGraphExecDestroy(c->graph_exec);
+            c->graph_exec = nullptr;
+        }
+        if (c->graph) {
+            (void)hipGraphDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        // buffers used inside the graph must exist before capture
+        BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
+        BD_TRY(ensure(c, (void **)&c->optics, (size_t)c->nvox * sizeof(float4)));
+        BD_TRY(ensure(c, (void **)&c->hlat, (size_t)c->nvox * 4));
+        const int64_t nd = beam->launch_axis == 0 ? c->g.nz : c->g.nx;
+        if (c->lat_n != nd) {
+            if (c->lat) BD_HIP(hipFree(c->lat));
+            c->lat = nullptr;
+            BD_HIP(hipMalloc((void **)&c->lat, sizeof(double) * nd));
+            c->lat_n = nd;
+        }
+        BD_HIP(hipStreamSynchronize(c->stream));
+        c->graph_T = c->T;
+        c->graph_P = c->P;
+        BD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_epoch(c, beam, n_substeps);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != BD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        BD_HIP(e);
+        c->graph = g;
+        BD_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_beam = *beam;
+        c->graph_nsub = n_substeps;
+    }
+    // per-epoch scalars -> device (outside the graph), replay, one readback
+    const double dt_epoch = n_substeps * c->phys.dt_sub;
+    EpochParams *hp = (EpochParams *)((char *)c->host_pinned + 56 * sizeof(double));
+    hp->seed_base = seed_base;
+    hp->epoch = (long long)epoch;
+    hp->t = t_before;
+    hp->dt_epoch = dt_epoch;
+    BD_HIP(hipEventRecord(c->ev2, c->stream));
+    BD_HIP(hipMemcpyAsync(c->ep, hp, sizeof(EpochParams), hipMemcpyHostToDevice, c->stream));
+    BD_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    BD_HIP(hipEventRecord(c->ev3, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    const double *r = (const double *)c->host_pinned;
+    const unsigned long long *cnt = (const unsigned long long *)(r + 48);
+    *out = bd_epoch_diag{};
+    const double V = c->g.dx * c->g.dy * c->g.dz;
+    if (beam->power > 0.0) {
+        const double N = (double)beam->n_photons;
+        const double *m = r + 16;
+        out->absorbed_frac = m[4] / N;
+        out->mc_energy_closure = (m[0] + m[1] + m[2] + m[3] + m[4]) / N;
+        out->reflected_J = (m[0] + m[1]) / N * beam->power * dt_epoch;
+        out->transmitted_J = m[2] / N * beam->power * dt_epoch;
+        out->lateral_J = m[3] / N * beam->power * dt_epoch;
+        out->R_spec = m[0] / N;
+        out->R_diff = m[1] / N;
+        out->transmission = m[2] / N;
+        out->lateral_loss = m[3] / N;
+        out->mc_events = m[5];
+        out->absorbed_J = r[40] * V * dt_epoch;
+    } else {
+        out->mc_energy_closure = 1.0;
+    }
+    out->conducted_J = -r[8];
+    out->ablated_J = r[9];
+    out->n_new = (int64_t)cnt[1];
+    if (c->plume.enabled) {
+        double d[4];
+        plume_diag_from(r + 24, d);
+        out->plume_tau = d[0];
+        out->plume_transmission = d[1];
+        out->plume_peak_mu_a = d[2];
+        out->plume_mean_mu_a = d[3];
+    } else {
+        out->plume_transmission = 1.0;
+    }
+    float ms = 0.f;
+    BD_HIP(hipEventElapsedTime(&ms, c->ev2, c->ev3));
+    out->t_thermal_ms = ms;  // whole replayed epoch (per-phase events are not recorded inside the graph)
+    return BD_OK;
+}

@@ -17,6 +17,25 @@ constexpr uint8_t F_ABL = 1;
 constexpr uint8_t F_DRY = 2;
 constexpr uint8_t F_HPOS = 4;
 
+// Per-epoch scalars kept on the device so a captured epoch graph can replay
+// without host updates: the MC key is SplitMix64(seed_base + (epoch+1) * gamma)
+// and the removal time is t + dt_epoch, advanced by a kernel at epoch end with
+// the same FP64 addition the host engine does (coupled_ablation.py:548).
+struct EpochParams {
+    unsigned long long seed_base;
+    long long epoch;
+    double t;
+    double dt_epoch;
+};
+
+__host__ __device__ inline unsigned long long mix_seed(unsigned long long seed, long long epoch)
+{
+    unsigned long long z = seed + (unsigned long long)(epoch + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 // Device buffers of a context.  Unused buffers stay null until first use.
 struct Field {
     void *ptr = nullptr;
@@ -71,6 +90,14 @@ struct Ctx {
     int tile_by = 4, tile_xc = 16;
     bool tuned = false;
     double last_ledger_conducted = 0.0;
+
+    // captured epoch graph (bd_epoch_graph) and the beam it was captured for
+    EpochParams *ep = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bd_beam graph_beam{};
+    int graph_nsub = -1;
+    const float *graph_T = nullptr, *graph_P = nullptr;  // ping-pong heads the graph was captured with
 };
 
 void set_error(const std::string &msg);
@@ -109,7 +136,13 @@ int launch_plateau(Ctx *c);
 int launch_arrhenius(Ctx *c, double A, double EaR, double dt);
 int launch_char(Ctx *c, double A, double EaR, double dt);
 int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out);
+// enqueue only (no host wait); with ep != null the photon key comes from the device params
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, bool timed);
 int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J);
+int launch_phase_enqueue(Ctx *c, double t_now, const EpochParams *ep);
+int launch_plume_diag_enqueue(Ctx *c);
+int reduce_masked_enqueue(Ctx *c, const float *src, int64_t n, const uint8_t *flags, uint8_t bit, bool invert,
+                          double *dev_out);
 int launch_plume_void(Ctx *c);
 int launch_plume_inject(Ctx *c);
 int launch_plume_sync(Ctx *c);

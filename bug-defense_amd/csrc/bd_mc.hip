@@ -47,6 +47,7 @@ struct McArgs {
     long long n_photons, photon_offset, max_events;
     unsigned k0, k1;
     int score_L;
+    const EpochParams *ep;            // when set, the photon key is mix_seed(seed_base, epoch)
 };
 
 constexpr unsigned TAG = 0x6d63u;
@@ -146,6 +147,12 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
     const int lane = threadIdx.x & 63;
     const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
     const int depth_axis = a.la == 0 ? 2 : 0;
+    unsigned key0 = a.k0, key1 = a.k1;
+    if (a.ep) {
+        const unsigned long long key = mix_seed(a.ep->seed_base, a.ep->epoch);
+        key0 = (unsigned)key;
+        key1 = (unsigned)(key >> 32);
+    }
     Photon p;
     Rng rng;
     bool alive = false;
@@ -198,8 +205,8 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
                 } else {
                     // ---- launch (voxel_mc_3d.py:150-194)
                     const unsigned long long pid = (unsigned long long)a.photon_offset + my;
-                    rng.k0 = a.k0;
-                    rng.k1 = a.k1;
+                    rng.k0 = key0;
+                    rng.k1 = key1;
                     rng.p0 = (unsigned)pid;
                     rng.p1 = (unsigned)(pid >> 32);
                     rng.blk = 0;
@@ -511,7 +518,7 @@ __global__ void k_q_from_a(const float2 *__restrict__ AL, float *__restrict__ Q,
         Q[i] = AL[i].x * scale;
 }
 
-int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, bool timed)
 {
     BD_TRY(launch_pack_optics(c, b->add_plume != 0));
     BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
@@ -553,11 +560,10 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
     double iu[3];
     if (b->launch_axis == 0) {
         iu[0] = sth * cph; iu[1] = sth * sph; iu[2] = cth;
-        a.cos_in_d = cth;
     } else {
         iu[0] = cth; iu[1] = sth * cph; iu[2] = sth * sph;
-        a.cos_in_d = cth;
     }
+    a.cos_in_d = cth;
     a.iu0 = (float)iu[0];
     a.iu1 = (float)iu[1];
     a.iu2 = (float)iu[2];
@@ -575,20 +581,27 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
     a.k0 = (unsigned)(b->seed & 0xffffffffu);
     a.k1 = (unsigned)(b->seed >> 32);
     a.score_L = b->score_fluence ? 1 : 0;
+    a.ep = ep;
 
     int ncu = 256;
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     const int64_t want = (b->n_photons + 255) / 256;
     const int64_t cap = (int64_t)ncu * 8;
     const int nblk = (int)(want < cap ? (want < 1 ? 1 : want) : cap);
-    BD_HIP(hipEventRecord(c->ev0, c->stream));
+    if (timed) BD_HIP(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_mc, dim3(nblk), dim3(256), 0, c->stream, a);
     BD_HIP(hipGetLastError());
-    BD_HIP(hipEventRecord(c->ev1, c->stream));
+    if (timed) BD_HIP(hipEventRecord(c->ev1, c->stream));
     const float scale = (float)(b->power / ((double)b->n_photons * c->g.dx * c->g.dy * c->g.dz));
     const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
     hipLaunchKernelGGL(k_q_from_a, dim3(nb), dim3(256), 0, c->stream, c->AL, c->Q, scale, c->nvox);
     BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
+{
+    BD_TRY(launch_mc_enqueue(c, b, nullptr, true));
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;
@@ -600,7 +613,7 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
     out->events = r[5];
     float ms = 0.f;
     BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    out->kernel_ms = ms;
+    out->kernel_ms = ms;  // the transport kernel alone
     return BD_OK;
 }
 

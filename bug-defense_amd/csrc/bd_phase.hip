@@ -40,6 +40,7 @@ struct PhaseArgs {
     float k_void, rc_void, t_void;
     double vol;
     int64_t nvox;
+    const EpochParams *ep;  // when set, t_now = ep->t + ep->dt_epoch (device-resident epoch time)
 };
 
 __device__ __forceinline__ float pf(const PParam &p, int64_t i) { return p.per_voxel ? p.f[i] : p.f[0]; }
@@ -49,6 +50,7 @@ __global__ __launch_bounds__(256) void k_phase(PhaseArgs a)
 {
     unsigned cnt = 0;
     double e_abl = 0.0;
+    const double t_now = a.ep ? a.ep->t + a.ep->dt_epoch : a.t_now;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.nvox; i += (int64_t)gridDim.x * blockDim.x) {
         uint8_t fl = a.flags[i];
         if (fl & F_ABL) {
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(256) void k_phase(PhaseArgs a)
             e_abl += a.vol * ((double)a.rc[i] * ((double)T - a.t_amb) + (double)h);
             fl = (uint8_t)((fl | F_ABL) & ~(F_DRY | F_HPOS));
             a.chi[i] = 0.f;
-            if (a.abl_time) a.abl_time[i] = a.t_now;
+            if (a.abl_time) a.abl_time[i] = t_now;
             a.mu_a[i] = 0.f;
             a.mu_s[i] = 0.f;
             a.g[i] = 0.f;
@@ -144,9 +146,10 @@ __global__ __launch_bounds__(256) void k_phase(PhaseArgs a)
 // not get [k0, k1) marked ablated at time t (only when cells were removed).
 __global__ void k_perforate(uint8_t *__restrict__ flags, double *__restrict__ abl_time,
                             const unsigned long long *__restrict__ n_new, int Nx, int Ny, int Nz, int k0, int k1,
-                            double t_now)
+                            double t_arg, const EpochParams *ep)
 {
     if (*n_new == 0) return;
+    const double t_now = ep ? ep->t + ep->dt_epoch : t_arg;
     const int64_t cols = (int64_t)Nx * Ny;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < cols; c += (int64_t)gridDim.x * blockDim.x) {
         const int64_t base = c * Nz;
@@ -159,7 +162,7 @@ __global__ void k_perforate(uint8_t *__restrict__ flags, double *__restrict__ ab
     }
 }
 
-int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J)
+int launch_phase_enqueue(Ctx *c, double t_now, const EpochParams *ep)
 {
     const bd_physics &p = c->phys;
     const size_t f = (size_t)c->nvox * 4;
@@ -231,6 +234,7 @@ int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J)
     a.rc_void = (float)p.rho_cp_void;
     a.t_void = (float)p.t_void;
     a.vol = c->g.dx * c->g.dy * c->g.dz;
+    a.ep = ep;
     a.nvox = c->nvox;
     BD_TRY(launch_fill(c, c->counter + 1, K_F64, 1, 0.0));
     BD_TRY(launch_fill(c, c->red + 9, K_F64, 1, 0.0));
@@ -240,9 +244,15 @@ int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J)
     if (!p.no_removal && p.air_gap_k0 > 0 && p.air_gap_k1 > p.air_gap_k0) {
         const int64_t cols = (int64_t)c->g.nx * c->g.ny;
         hipLaunchKernelGGL(k_perforate, dim3(blocks_for(cols, 256)), dim3(256), 0, c->stream, c->flags, c->abl_time,
-                           c->counter + 1, c->g.nx, c->g.ny, c->g.nz, p.air_gap_k0, p.air_gap_k1, t_now);
+                           c->counter + 1, c->g.nx, c->g.ny, c->g.nz, p.air_gap_k0, p.air_gap_k1, t_now, ep);
         BD_HIP(hipGetLastError());
     }
+    return BD_OK;
+}
+
+int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J)
+{
+    BD_TRY(launch_phase_enqueue(c, t_now, nullptr));
     if (n_new) {
         BD_HIP(hipMemcpyAsync(c->host_pinned, c->counter + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               c->stream));
@@ -254,5 +264,6 @@ int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J)
     }
     return BD_OK;
 }
+
 
 }  // namespace bd

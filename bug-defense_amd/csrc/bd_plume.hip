@@ -182,6 +182,8 @@ __global__ void k_splat1(const uint8_t *__restrict__ rem, const float *__restric
                          int ym, int zm, int Lx, int Ly, int Lz, const float *__restrict__ kx, int nkx,
                          float *__restrict__ t1)
 {
+    // the reference sums tap by tap with separate multiply and add roundings: no FMA contraction
+#pragma clang fp contract(off)
     const int OX = Lx + nkx - 1;
     const int64_t N = (int64_t)OX * Ly * Lz;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x) {
@@ -204,6 +206,8 @@ __global__ void k_splat1(const uint8_t *__restrict__ rem, const float *__restric
 __global__ void k_splat2(const float *__restrict__ t1, int OX, int Ly, int Lz, const float *__restrict__ ky, int nky,
                          float *__restrict__ t2)
 {
+    // the reference sums tap by tap with separate multiply and add roundings: no FMA contraction
+#pragma clang fp contract(off)
     const int OY = Ly + nky - 1;
     const int64_t N = (int64_t)OX * OY * Lz;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x) {
@@ -225,6 +229,8 @@ __global__ void k_splat2(const float *__restrict__ t1, int OX, int Ly, int Lz, c
 __global__ void k_splat3(const float *__restrict__ t2, int OX, int OY, int Lz, const float *__restrict__ kz, int nkz,
                          double delta, int x0, int y0, int z0, float *__restrict__ P, int Nx, int Ny, int Nz)
 {
+    // the reference sums tap by tap with separate multiply and add roundings: no FMA contraction
+#pragma clang fp contract(off)
     const int OZ = Lz + nkz - 1;
     const int64_t N = (int64_t)OX * OY * OZ;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x) {
@@ -440,7 +446,7 @@ __global__ void k_plume_diag(const float *__restrict__ E, const uint8_t *__restr
     }
 }
 
-int launch_plume_diag(Ctx *c, double out[4])
+int launch_plume_diag_enqueue(Ctx *c)
 {
     const bd_plume_params &p = c->plume;
     double *acc = c->red + 24;
@@ -454,17 +460,27 @@ int launch_plume_diag(Ctx *c, double out[4])
     hipLaunchKernelGGL(k_plume_diag, dim3(ew_blocks(std::max(c->pvox, c->nvox))), dim3(256), 0, c->stream, c->Pext,
                        c->flags, c->g.nx, c->g.ny, c->g.nz, p.nx, p.ny, p.nz, ra, I, J, dz, acc, pk);
     BD_HIP(hipGetLastError());
-    BD_HIP(hipMemcpyAsync(c->host_pinned, acc, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    BD_HIP(hipStreamSynchronize(c->stream));
-    const double *h = (const double *)c->host_pinned;
+    return BD_OK;
+}
+
+void plume_diag_from(const double *h, double out[4])
+{
     unsigned bits;
-    std::memcpy(&bits, (const char *)c->host_pinned + 24, 4);
+    std::memcpy(&bits, (const char *)(h + 3), 4);
     float peak;
     std::memcpy(&peak, &bits, 4);
     out[0] = h[0];
     out[1] = std::exp(-h[0]);
     out[2] = (double)peak;
     out[3] = h[2] > 0.0 ? h[1] / h[2] : 0.0;
+}
+
+int launch_plume_diag(Ctx *c, double out[4])
+{
+    BD_TRY(launch_plume_diag_enqueue(c));
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 24, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    plume_diag_from((const double *)c->host_pinned, out);
     return BD_OK;
 }
 

@@ -155,8 +155,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__re
     if (threadIdx.x == 0) *out = s[0];
 }
 
-static int reduce_impl(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t bits,
-                       int invert, double *out)
+static int reduce_enqueue(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t bits,
+                          int invert, double *dev_out)
 {
     // partials live in the staging tail of the reduction scratch
     double *partial = nullptr;
@@ -169,8 +169,15 @@ static int reduce_impl(Ctx *c, const void *src, int kind, int64_t n, int op, con
     default: hipLaunchKernelGGL(k_reduce_partial<K_U8>, gr, bl, 0, c->stream, src, n, op, mask, bits, invert, partial); break;
     }
     BD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), bl, 0, c->stream, partial, RED_BLOCKS, op, c->red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), bl, 0, c->stream, partial, RED_BLOCKS, op, dev_out);
     BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+static int reduce_impl(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t bits,
+                       int invert, double *out)
+{
+    BD_TRY(reduce_enqueue(c, src, kind, n, op, mask, bits, invert, c->red));
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     *out = *(double *)c->host_pinned;
@@ -181,6 +188,12 @@ int launch_reduce(Ctx *c, const void *src, int kind, int64_t n, int op, const ui
                   double *out)
 {
     return reduce_impl(c, src, kind, n, op, mask, bits, 0, out);
+}
+
+int reduce_masked_enqueue(Ctx *c, const float *src, int64_t n, const uint8_t *flags, uint8_t bit, bool invert,
+                          double *dev_out)
+{
+    return reduce_enqueue(c, src, K_F32, n, 0, flags, bit, invert ? 1 : 0, dev_out);
 }
 
 int reduce_masked(Ctx *c, const float *src, int64_t n, const uint8_t *flags, uint8_t bit, bool invert, double *out)

@@ -125,6 +125,8 @@ _SIGS = {
     "bd_plume_diag": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bd_epoch": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double,
                   ctypes.POINTER(EpochDiag)], ctypes.c_int),
+    "bd_epoch_graph": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double, ctypes.c_int64,
+                        ctypes.c_uint64, ctypes.POINTER(EpochDiag)], ctypes.c_int),
     "bd_reduce": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                   ctypes.c_int),
     "bd_phase_map": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
@@ -332,6 +334,12 @@ class Context:
         d = EpochDiag()
         check(lib().bd_epoch(self.handle, ctypes.byref(beam), int(n_substeps), float(t_after), ctypes.byref(d)),
               "bd_epoch")
+        return d
+
+    def epoch_graph(self, beam: Beam, n_substeps, t_before, epoch, seed_base) -> EpochDiag:
+        d = EpochDiag()
+        check(lib().bd_epoch_graph(self.handle, ctypes.byref(beam), int(n_substeps), float(t_before), int(epoch),
+                                   int(seed_base) & 0xFFFFFFFFFFFFFFFF, ctypes.byref(d)), "bd_epoch_graph")
         return d
 
     def reduce(self, name, op="sum", mask=None):

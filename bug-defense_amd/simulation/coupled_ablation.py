@@ -20,8 +20,9 @@ uploaded back before the next epoch, because callers mutate it in place
 
 Extra keywords (not in the reference): ``seed`` (Philox key of the MC
 streams; random when None), ``plume_grid`` (shape of a separate plume grid,
-default the thermal grid) and ``score_fluence`` (path-length tally for
-``Phi``; default True like the reference).
+default the thermal grid), ``score_fluence`` (path-length tally for ``Phi``;
+default True like the reference) and ``graph`` (replay each epoch from a
+captured hipGraph when the configuration allows it).
 """
 
 from __future__ import annotations
@@ -141,7 +142,7 @@ class CoupledAblationEngine:
                  beam_profile='gaussian', dry_state=None, char_state=None, plume_closure=None, theta_inc_deg=0.0,
                  phi_inc_deg=0.0, initial_dry_mask=None, initial_ablated_mask=None, air_gap_range=None,
                  wavelength_nm=10600.0, mc_engine='auto', thermal_engine='auto', seed=None, plume_grid=None,
-                 score_fluence=True):
+                 score_fluence=True, graph=False):
         check_engine(mc_engine)
         check_engine(thermal_engine)
         self.__dict__['_cache'] = {}
@@ -169,6 +170,7 @@ class CoupledAblationEngine:
         self.use_gpu_thermal = True
         self.seed = int(seed) if seed is not None else int(np.random.randint(1, 2 ** 62))
         self.score_fluence = bool(score_fluence)
+        self.graph = bool(graph)
         shape = (self.Nx, self.Ny, self.Nz)
         self._ctx = B.Context(shape, (self.dx, self.dy, self.dz))
         ctx = self._ctx
@@ -436,7 +438,14 @@ class CoupledAblationEngine:
                          score_fluence=self.score_fluence, add_plume=self.plume is not None)
         t_after = self.t + self.dt_epoch
         t0 = time.perf_counter()
-        d = self._ctx.epoch(beam, self.n_substeps, t_after)
+        graphable = (self.graph and self.n_substeps % 2 == 0 and abs(self.n_substeps * self.dt_sub - self.dt_epoch)
+                     <= 1e-12 * self.dt_epoch and (self.plume is None or (self.plume.model == 'diffusion'
+                                                                          and not self.plume.blast_expansion)))
+        if graphable:
+            # same MC key (mix_seed(seed, epoch)) and removal time (t + dt_epoch), formed on the device
+            d = self._ctx.epoch_graph(beam, self.n_substeps, self.t, self.epoch, self.seed)
+        else:
+            d = self._ctx.epoch(beam, self.n_substeps, t_after)
         wall = time.perf_counter() - t0
         self._invalidate()
         self.t = t_after

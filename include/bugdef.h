@@ -212,6 +212,17 @@ int bd_plume_diag(bd_ctx *ctx, double out[4]); /* tau, transmission, peak, mean 
  * injection/sync and diagnostics, with FP64 energy ledgers. */
 int bd_epoch(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_after, bd_epoch_diag *out);
 
+/* The same epoch replayed from a captured hipGraph: no host wait inside the
+ * epoch, one parameter upload and one diagnostics readback per call.  The MC
+ * key is SplitMix64(seed_base + (epoch + 1) * 0x9E3779B97F4A7C15) and the
+ * removal time t_before + n_substeps * dt_sub, so a run stepped with this call
+ * equals one stepped with bd_epoch at beam->seed = that key.  Requires an even
+ * n_substeps and a diffusion plume without blast injection (or no plume); the
+ * graph is re-captured when the beam or n_substeps change.  Per-phase timings
+ * are not recorded: t_thermal_ms holds the whole epoch. */
+int bd_epoch_graph(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_before, int64_t epoch,
+                   uint64_t seed_base, bd_epoch_diag *out);
+
 /* Reductions (FP64): op 0 sum, 1 max, 2 min over a field; optional u8 mask
  * field (-1 for none) restricts the reduction to mask != 0. */
 int bd_reduce(bd_ctx *ctx, int field, int op, int mask_field, double *out);

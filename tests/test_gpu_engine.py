@@ -228,3 +228,24 @@ def test_engine_against_oracle_engine(oracle):
     # removed-volume and char agree statistically
     assert abs(int(eng.ablated.sum()) - int(ref.ablated.sum())) <= 0.2 * max(10, int(ref.ablated.sum() - abl.sum()))
     assert eng.T_peak.max() >= float(Tg.max()) - 1e-4
+
+
+def test_graph_epochs_match_eager_epochs():
+    """bd_epoch_graph (captured epoch, device-side key and time) against bd_epoch: same seed,
+    same photon streams; the only difference is float-atomic summation order in the tallies."""
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.workloads import layered_stack
+    kw = layered_stack(shape=(48, 48, 64), n_photons=30000, plume_grid=(24, 24, 32), power=60.0)
+    e1 = CoupledAblationEngine(graph=False, **kw)
+    e2 = CoupledAblationEngine(graph=True, **layered_stack(shape=(48, 48, 64), n_photons=30000,
+                                                           plume_grid=(24, 24, 32), power=60.0))
+    for ep in range(8):
+        if ep == 6:
+            e1.power = e2.power = 0.0  # laser off: a second captured graph
+        d1, d2 = e1.step_epoch(), e2.step_epoch()
+        assert d1['absorbed_frac'] == pytest.approx(d2['absorbed_frac'], abs=1e-6)
+        assert abs(d1['n_new'] - d2['n_new']) <= max(2, 0.02 * d1['n_new'])
+    T1, T2 = np.asarray(e1.T), np.asarray(e2.T)
+    assert float(np.max(np.abs(T1 - T2))) < 1e-2 * float(T1.max() - 20.0) + 1e-3
+    np.testing.assert_allclose(e1.ablation_time[e1.ablated & e2.ablated], e2.ablation_time[e1.ablated & e2.ablated])
+    assert e1.t == e2.t and e1.epoch == e2.epoch == 8
