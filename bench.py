@@ -51,6 +51,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-epochs", type=int, default=3)
+    ap.add_argument("--graph", action="store_true", help="replay each epoch as a captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
@@ -66,7 +67,7 @@ def main():
         raise RuntimeError("bench: no HIP device visible")
     kw = layered_stack()
     t0 = time.perf_counter()
-    eng = CoupledAblationEngine(**kw)
+    eng = CoupledAblationEngine(graph=args.graph, **kw)
     log(f"bench: engine built in {time.perf_counter() - t0:.1f}s ({eng.Nx}x{eng.Ny}x{eng.Nz})")
     for i in range(args.warmup):
         eng.step_epoch()

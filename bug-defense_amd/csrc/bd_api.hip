@@ -139,6 +139,7 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     BD_HIP(hipEventCreate(&c->ev3));
     BD_HIP(hipEventCreate(&c->ev4));
     BD_HIP(hipEventCreate(&c->ev5));
+    for (int i = 0; i < 6; ++i) BD_HIP(hipEventCreate(&c->gev[i]));
     BD_HIP(hipHostMalloc(&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
     BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + 1024)));
     BD_TRY(ensure(c, (void **)&c->counter, sizeof(unsigned long long) * 16));
@@ -171,6 +172,8 @@ int bd_destroy(bd_ctx *h)
     if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->ev4) (void)hipEventDestroy(c->ev4);
     if (c->ev5) (void)hipEventDestroy(c->ev5);
+    for (int i = 0; i < 6; ++i)
+        if (c->gev[i]) (void)hipEventDestroy(c->gev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete h;
     return BD_OK;

@@ -139,7 +139,7 @@ static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
     const bool plume = c->plume.enabled != 0;
     const bool diff_plume = plume && c->plume.model == 0;
     if (beam->power > 0.0) {
-        BD_TRY(launch_mc_enqueue(c, beam, c->ep, false));
+        BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2));
         BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
     } else {
         BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
@@ -147,17 +147,21 @@ static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
         BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
         BD_TRY(launch_fill(c, c->red + 40, K_F64, 1, 0.0));
     }
+    BD_HIP(hipEventRecordWithFlags(c->gev[2], c->stream, hipEventRecordExternal));
     if (diff_plume) BD_TRY(launch_plume_void(c));
     int flags = BD_SUB_PLATEAU | BD_SUB_MASK_Q | BD_SUB_PEAK;
     if (p.has_arrhenius) flags |= BD_SUB_ARRHENIUS;
     if (p.has_char) flags |= BD_SUB_CHAR;
     BD_TRY(launch_fill(c, c->red + 8, K_F64, 1, 0.0));
+    BD_HIP(hipEventRecordWithFlags(c->gev[3], c->stream, hipEventRecordExternal));
     BD_TRY(launch_substeps(c, n_substeps, 1, flags, true));
+    BD_HIP(hipEventRecordWithFlags(c->gev[4], c->stream, hipEventRecordExternal));
     if (diff_plume) {
         const double tl = c->plume.tau_linger;
         const double decay = (tl > 0.0 && std::isfinite(tl)) ? std::exp(-p.dt_sub / tl) : 1.0;
         BD_TRY(launch_plume_substeps(c, n_substeps, p.dt_sub, decay));
     }
+    BD_HIP(hipEventRecordWithFlags(c->gev[5], c->stream, hipEventRecordExternal));
     BD_TRY(launch_phase_enqueue(c, 0.0, c->ep));
     if (plume) {
         BD_TRY(launch_plume_void(c));
@@ -293,8 +297,19 @@ extern "C" int bd_epoch_graph(bd_ctx *h, const bd_beam *beam, int n_substeps, do
     } else {
         out->plume_transmission = 1.0;
     }
+    // phase spans from the external event nodes recorded inside the graph
     float ms = 0.f;
-    BD_HIP(hipEventElapsedTime(&ms, c->ev2, c->ev3));
-    out->t_thermal_ms = ms;  // whole replayed epoch (per-phase events are not recorded inside the graph)
+    BD_HIP(hipEventElapsedTime(&ms, c->ev2, c->gev[2]));
+    out->t_optical_ms = ms;
+    BD_HIP(hipEventElapsedTime(&ms, c->gev[2], c->gev[5]));
+    out->t_thermal_ms = ms;
+    BD_HIP(hipEventElapsedTime(&ms, c->gev[5], c->ev3));
+    out->t_phase_ms = ms;
+    BD_HIP(hipEventElapsedTime(&ms, c->gev[3], c->gev[4]));
+    out->t_substeps_ms = ms;
+    if (beam->power > 0.0) {
+        BD_HIP(hipEventElapsedTime(&ms, c->gev[0], c->gev[1]));
+        out->t_mc_kernel_ms = ms;
+    }
     return BD_OK;
 }

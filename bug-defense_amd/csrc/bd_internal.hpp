@@ -49,6 +49,9 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
+    // recorded as external event nodes inside the captured epoch graph: MC kernel start/end,
+    // optics end, sub-steps start/end, epoch end
+    hipEvent_t gev[6] = {};
     bd_physics phys{};
     bool has_phys = false;
     bd_plume_params plume{};
@@ -137,7 +140,8 @@ int launch_arrhenius(Ctx *c, double A, double EaR, double dt);
 int launch_char(Ctx *c, double A, double EaR, double dt);
 int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out);
 // enqueue only (no host wait); with ep != null the photon key comes from the device params
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, bool timed);
+// timed: 0 no events, 1 ev0/ev1 around the transport kernel, 2 graph-external gev[0]/gev[1]
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed);
 int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J);
 int launch_phase_enqueue(Ctx *c, double t_now, const EpochParams *ep);
 int launch_plume_diag_enqueue(Ctx *c);

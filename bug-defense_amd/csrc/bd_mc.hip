@@ -518,7 +518,7 @@ __global__ void k_q_from_a(const float2 *__restrict__ AL, float *__restrict__ Q,
         Q[i] = AL[i].x * scale;
 }
 
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, bool timed)
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed)
 {
     BD_TRY(launch_pack_optics(c, b->add_plume != 0));
     BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
@@ -588,10 +588,12 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, bool time
     const int64_t want = (b->n_photons + 255) / 256;
     const int64_t cap = (int64_t)ncu * 8;
     const int nblk = (int)(want < cap ? (want < 1 ? 1 : want) : cap);
-    if (timed) BD_HIP(hipEventRecord(c->ev0, c->stream));
+    if (timed == 1) BD_HIP(hipEventRecord(c->ev0, c->stream));
+    if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[0], c->stream, hipEventRecordExternal));
     hipLaunchKernelGGL(k_mc, dim3(nblk), dim3(256), 0, c->stream, a);
     BD_HIP(hipGetLastError());
-    if (timed) BD_HIP(hipEventRecord(c->ev1, c->stream));
+    if (timed == 1) BD_HIP(hipEventRecord(c->ev1, c->stream));
+    if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[1], c->stream, hipEventRecordExternal));
     const float scale = (float)(b->power / ((double)b->n_photons * c->g.dx * c->g.dy * c->g.dz));
     const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
     hipLaunchKernelGGL(k_q_from_a, dim3(nb), dim3(256), 0, c->stream, c->AL, c->Q, scale, c->nvox);
@@ -601,7 +603,7 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, bool time
 
 int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
 {
-    BD_TRY(launch_mc_enqueue(c, b, nullptr, true));
+    BD_TRY(launch_mc_enqueue(c, b, nullptr, 1));
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;

@@ -408,22 +408,24 @@ int launch_plume_update_sep(Ctx *c, double dt_epoch)
 }
 
 // coupled_ablation.py:594-603: centre-column optical depth, peak, mean over ablated cells
-__global__ void k_plume_diag(const float *__restrict__ E, const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz,
-                             int Px, int Py, int Pz, float ra, int I, int J, double dzp, double *__restrict__ acc,
-                             unsigned *__restrict__ peak_bits)
+// stage 1 of the centre-column / ablated-mean / peak diagnostics
+// (coupled_ablation.py:594-603): per-block partial sum and count over ablated
+// voxels of the plume value at the voxel's plume cell, and the block's peak bits;
+// one write per block, no contended atomics
+constexpr int PD_BLOCKS = 256, PD_THREADS = 256;
+__global__ __launch_bounds__(PD_THREADS) void k_plume_diag1(const float *__restrict__ E,
+                                                            const uint8_t *__restrict__ flags, int Nx, int Ny,
+                                                            int Nz, int Px, int Py, int Pz, float ra,
+                                                            double *__restrict__ part)
 {
-    // acc[0] column tau, acc[1] sum over ablated, acc[2] count ablated
     const int64_t NP = (int64_t)Px * Py * Pz;
     const int64_t N = (int64_t)Nx * Ny * Nz;
-    double col = 0.0, s = 0.0, cnt = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    double s = 0.0, cnt = 0.0;
     unsigned pk = 0;
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < NP; c += (int64_t)gridDim.x * blockDim.x) {
-        const float m = E[c] * ra;
-        pk = max(pk, __float_as_uint(m));
-        const int64_t r = c / Pz;
-        if ((int)(r % Py) == J && (int)(r / Py) == I) col += (double)m * dzp;
-    }
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < NP; c += stride)
+        pk = max(pk, __float_as_uint(E[c] * ra));
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += stride) {
         if (!(flags[c] & F_ABL)) continue;
         const int k = (int)(c % Nz);
         const int64_t r = c / Nz;
@@ -432,17 +434,65 @@ __global__ void k_plume_diag(const float *__restrict__ E, const uint8_t *__restr
         s += (double)(E[pc] * ra);
         cnt += 1.0;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        col += __shfl_down(col, off);
-        s += __shfl_down(s, off);
-        cnt += __shfl_down(cnt, off);
-        pk = max(pk, (unsigned)__shfl_down((int)pk, off));
+    __shared__ double ss[PD_THREADS], sc[PD_THREADS];
+    __shared__ unsigned sp[PD_THREADS];
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = cnt;
+    sp[threadIdx.x] = pk;
+    __syncthreads();
+    for (int h = PD_THREADS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            ss[threadIdx.x] += ss[threadIdx.x + h];
+            sc[threadIdx.x] += sc[threadIdx.x + h];
+            sp[threadIdx.x] = max(sp[threadIdx.x], sp[threadIdx.x + h]);
+        }
+        __syncthreads();
     }
-    if ((threadIdx.x & 63) == 0) {
-        if (col != 0.0) unsafeAtomicAdd(&acc[0], col);
-        if (s != 0.0) unsafeAtomicAdd(&acc[1], s);
-        if (cnt != 0.0) unsafeAtomicAdd(&acc[2], cnt);
-        if (pk) atomicMax(peak_bits, pk);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = ss[0];
+        part[3 * blockIdx.x + 1] = sc[0];
+        part[3 * blockIdx.x + 2] = (double)sp[0];  // exact: 32-bit value in a double
+    }
+}
+
+// stage 2 (one block): fixed-order sums of the partials and the column tau
+// summed along z in order; acc = {tau, sum, count, peak bits}
+__global__ __launch_bounds__(PD_THREADS) void k_plume_diag2(const float *__restrict__ E, const double *__restrict__ part,
+                                                            int nb, int Py, int Pz, float ra, int I, int J,
+                                                            double dzp, double *__restrict__ acc)
+{
+    __shared__ double ss[PD_THREADS], sc[PD_THREADS];
+    __shared__ unsigned sp[PD_THREADS];
+    double s = 0.0, cnt = 0.0;
+    unsigned pk = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        s += part[3 * b];
+        cnt += part[3 * b + 1];
+        pk = max(pk, (unsigned)part[3 * b + 2]);
+    }
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = cnt;
+    sp[threadIdx.x] = pk;
+    __syncthreads();
+    for (int h = PD_THREADS / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            ss[threadIdx.x] += ss[threadIdx.x + h];
+            sc[threadIdx.x] += sc[threadIdx.x + h];
+            sp[threadIdx.x] = max(sp[threadIdx.x], sp[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float *col = E + ((int64_t)I * Py + J) * Pz;
+        double tau = 0.0;
+        for (int k = 0; k < Pz; ++k) tau += (double)(col[k] * ra) * dzp;
+        acc[0] = tau;
+        acc[1] = ss[0];
+        acc[2] = sc[0];
+        unsigned bits = sp[0];
+        double packed = 0.0;
+        __builtin_memcpy(&packed, &bits, 4);
+        acc[3] = packed;
     }
 }
 
@@ -450,15 +500,17 @@ int launch_plume_diag_enqueue(Ctx *c)
 {
     const bd_plume_params &p = c->plume;
     double *acc = c->red + 24;
-    unsigned *pk = (unsigned *)(c->red + 27);
-    BD_TRY(launch_fill(c, acc, K_F64, 4, 0.0));
+    double *part = c->red + 64;  // the reduction partials (3 x PD_BLOCKS <= 1024 doubles)
     double dx, dy, dz;
     plume_spacing(c, dx, dy, dz);
     const int I = (int)(((int64_t)(2 * (c->g.nx / 2) + 1) * p.nx) / (2 * (int64_t)c->g.nx));
     const int J = (int)(((int64_t)(2 * (c->g.ny / 2) + 1) * p.ny) / (2 * (int64_t)c->g.ny));
     const float ra = (float)(p.mu_a_max / std::fmax(1e-6, p.mu_max));
-    hipLaunchKernelGGL(k_plume_diag, dim3(ew_blocks(std::max(c->pvox, c->nvox))), dim3(256), 0, c->stream, c->Pext,
-                       c->flags, c->g.nx, c->g.ny, c->g.nz, p.nx, p.ny, p.nz, ra, I, J, dz, acc, pk);
+    hipLaunchKernelGGL(k_plume_diag1, dim3(PD_BLOCKS), dim3(PD_THREADS), 0, c->stream, c->Pext, c->flags, c->g.nx,
+                       c->g.ny, c->g.nz, p.nx, p.ny, p.nz, ra, part);
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_plume_diag2, dim3(1), dim3(PD_THREADS), 0, c->stream, c->Pext, part, PD_BLOCKS, p.ny, p.nz,
+                       ra, I, J, dz, acc);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }

@@ -1,0 +1,206 @@
+"""FP64 NumPy restatement of the absorbed-power surrogate network (3-D Fourier
+neural operator) -- test infrastructure only.
+
+Only the tests, the smoke check and the benchmark's CPU baseline may import this
+module, and only as the checker.  The device network lives in
+``bug-defense_amd/csrc/bd_fno.hip``.
+
+The reference has no learned component; this network is the one the device
+library trains and runs, restated here so its forward pass and every gradient
+can be checked.  Pins that do not depend on this file's own formulas:
+
+* the truncated transforms equal ``numpy.fft`` (zero-padded to ``Np``) restricted
+  to the kept modes (``test_fno_oracle.py``),
+* every parameter gradient of :func:`backward` matches central finite
+  differences of :func:`forward` on a small tile,
+* a network whose spectral weights are zero reduces to a per-voxel MLP
+  evaluated independently with plain matrix products.
+
+Architecture (channels last, tile ``T^3``):
+
+    v0 = x W_lift + b_lift                                   (Cin -> C)
+    v_{l+1} = act_l( v_l W_l + b_l + S_l(v_l) ),  l = 0..L-1  (act = GELU, identity for the last)
+    y = GELU(v_L P1 + c1) p2 + c2                            (C -> H -> 1)
+
+``S_l`` is the spectral convolution: forward transforms along z (kz = 0..Kz-1),
+y (Ky signed modes) and x (Kx signed modes) with basis exp(-2 pi i k n / Np),
+a complex channel mix per kept mode, and the inverse transforms with
+exp(+2 pi i k n / Np); the z inverse takes the real part with weight 1 for
+kz = 0 and 2 otherwise, all scaled by 1/Np^3.  ``Np >= T`` zero-pads the
+transform so the convolution does not wrap around the tile.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+from scipy.special import erf
+
+__all__ = ["Spec", "param_shapes", "n_params", "unflatten", "tables", "forward", "backward", "gelu", "gelu_grad",
+           "init_params", "loss_and_grad"]
+
+
+@dataclass(frozen=True)
+class Spec:
+    T: int = 64        # tile edge (voxels)
+    Kx: int = 8        # kept signed modes along x (Kx/2 non-negative, Kx/2 negative)
+    Ky: int = 8
+    Kz: int = 4        # kept non-negative modes along z
+    Np: int = 80       # transform length (zero padding beyond T)
+    C: int = 32        # width
+    L: int = 4         # Fourier layers
+    H: int = 64        # projection hidden width
+    Cin: int = 8       # input channels
+
+
+def param_shapes(s: Spec):
+    """Ordered (name, shape) of the flat parameter vector (device layout)."""
+    out = [("lift_w", (s.Cin, s.C)), ("lift_b", (s.C,))]
+    for l in range(s.L):
+        out += [(f"spec{l}", (s.Kx, s.Ky, s.Kz, s.C, s.C, 2)), (f"w{l}", (s.C, s.C)), (f"b{l}", (s.C,))]
+    out += [("p1", (s.C, s.H)), ("c1", (s.H,)), ("p2", (s.H,)), ("c2", (1,))]
+    return out
+
+
+def n_params(s: Spec) -> int:
+    return int(sum(np.prod(sh) for _, sh in param_shapes(s)))
+
+
+def unflatten(s: Spec, flat):
+    p, o = {}, 0
+    for name, sh in param_shapes(s):
+        n = int(np.prod(sh))
+        p[name] = np.asarray(flat[o:o + n], dtype=np.float64).reshape(sh)
+        o += n
+    assert o == len(flat)
+    return p
+
+
+def flatten(s: Spec, p):
+    return np.concatenate([np.asarray(p[name], dtype=np.float64).ravel() for name, _ in param_shapes(s)])
+
+
+def init_params(s: Spec, seed=0):
+    """Deterministic initialisation (fan-in scaled normals, small spectral weights)."""
+    rng = np.random.default_rng(seed)
+    p = {}
+    for name, sh in param_shapes(s):
+        if name.startswith("spec"):
+            p[name] = rng.standard_normal(sh) * (1.0 / (s.C * s.C)) ** 0.5 * 0.5
+        elif name in ("lift_w", "p1") or (name[0] == "w" and name[1:].isdigit()):
+            p[name] = rng.standard_normal(sh) / np.sqrt(sh[0])
+        elif name == "p2":
+            p[name] = rng.standard_normal(sh) / np.sqrt(sh[0])
+        else:
+            p[name] = np.zeros(sh)
+    return flatten(s, p)
+
+
+def signed_modes(K):
+    m = K // 2
+    return np.array(list(range(m)) + list(range(-m, 0)), dtype=np.int64)
+
+
+def tables(s: Spec):
+    n = np.arange(s.T)
+    Ex = np.exp(-2j * np.pi * np.outer(signed_modes(s.Kx), n) / s.Np)   # [Kx, T]
+    Ey = np.exp(-2j * np.pi * np.outer(signed_modes(s.Ky), n) / s.Np)   # [Ky, T]
+    kz = np.arange(s.Kz)
+    Ez = np.exp(-2j * np.pi * np.outer(kz, n) / s.Np)                   # [Kz, T]
+    cz = np.where(kz == 0, 1.0, 2.0)
+    Gz = cz[:, None] * np.exp(2j * np.pi * np.outer(kz, n) / s.Np) / float(s.Np) ** 3   # [Kz, T]
+    return Ex, Ey, Ez, Gz
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+
+
+def gelu_grad(x):
+    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+
+
+def _spectral(s, tb, spec, v):
+    Ex, Ey, Ez, Gz = tb
+    Z = np.einsum('kz,bxyzc->bxykc', Ez, v)
+    Yf = np.einsum('qy,bxykc->bxqkc', Ey, Z)
+    Xf = np.einsum('px,bxqkc->bpqkc', Ex, Yf)
+    W = spec[..., 0] + 1j * spec[..., 1]
+    M = np.einsum('pqkio,bpqki->bpqko', W, Xf)
+    Xi = np.einsum('px,bpqko->bxqko', np.conj(Ex), M)
+    U = np.einsum('qy,bxqko->bxyko', np.conj(Ey), Xi)
+    S = np.real(np.einsum('kz,bxyko->bxyzo', Gz, U))
+    return S, Xf
+
+
+def forward(s: Spec, flat, x, keep=False):
+    """x: [B, T, T, T, Cin] -> y: [B, T, T, T] (float64)."""
+    p = unflatten(s, flat)
+    tb = tables(s)
+    x = np.asarray(x, dtype=np.float64)
+    v = x @ p["lift_w"] + p["lift_b"]
+    cache = {"x": x, "v": [v], "pre": [], "Xf": []}
+    for l in range(s.L):
+        S, Xf = _spectral(s, tb, p[f"spec{l}"], v)
+        pre = v @ p[f"w{l}"] + p[f"b{l}"] + S
+        v = gelu(pre) if l < s.L - 1 else pre
+        cache["pre"].append(pre)
+        cache["Xf"].append(Xf)
+        cache["v"].append(v)
+    hpre = v @ p["p1"] + p["c1"]
+    y = gelu(hpre) @ p["p2"] + p["c2"][0]
+    cache["hpre"] = hpre
+    return (y, cache) if keep else y
+
+
+def backward(s: Spec, flat, cache, gy):
+    """Gradient of sum(gy * y) with respect to the flat parameters."""
+    p = unflatten(s, flat)
+    Ex, Ey, Ez, Gz = tables(s)
+    g = {}
+    gy = np.asarray(gy, dtype=np.float64)
+    hpre = cache["hpre"]
+    h = gelu(hpre)
+    vL = cache["v"][-1]
+    g["c2"] = np.array([gy.sum()])
+    g["p2"] = h.reshape(-1, s.H).T @ gy.reshape(-1)
+    ghp = gy[..., None] * p["p2"] * gelu_grad(hpre)
+    g["p1"] = vL.reshape(-1, s.C).T @ ghp.reshape(-1, s.H)
+    g["c1"] = ghp.reshape(-1, s.H).sum(0)
+    gv = ghp @ p["p1"].T
+    for l in reversed(range(s.L)):
+        vl, pre, Xf = cache["v"][l], cache["pre"][l], cache["Xf"][l]
+        gpre = gv * gelu_grad(pre) if l < s.L - 1 else gv
+        g[f"w{l}"] = vl.reshape(-1, s.C).T @ gpre.reshape(-1, s.C)
+        g[f"b{l}"] = gpre.reshape(-1, s.C).sum(0)
+        gv_new = gpre @ p[f"w{l}"].T
+        # adjoint of S = Re(Gz U): gU = conj(Gz) gS (complex gradient convention d/dre + i d/dim)
+        gU = np.einsum('kz,bxyzo->bxyko', np.conj(Gz), gpre)
+        gXi = np.einsum('qy,bxyko->bxqko', Ey, gU)
+        gM = np.einsum('px,bxqko->bpqko', Ex, gXi)
+        W = p[f"spec{l}"][..., 0] + 1j * p[f"spec{l}"][..., 1]
+        gW = np.einsum('bpqki,bpqko->pqkio', np.conj(Xf), gM)
+        g[f"spec{l}"] = np.stack([gW.real, gW.imag], axis=-1)
+        gXf = np.einsum('pqkio,bpqko->bpqki', np.conj(W), gM)
+        gYf = np.einsum('px,bpqkc->bxqkc', np.conj(Ex), gXf)
+        gZ = np.einsum('qy,bxqkc->bxykc', np.conj(Ey), gYf)
+        gv_spec = np.real(np.einsum('kz,bxykc->bxyzc', np.conj(Ez), gZ))
+        gv = gv_new + gv_spec
+    x = cache["x"]
+    g["lift_w"] = x.reshape(-1, s.Cin).T @ gv.reshape(-1, s.C)
+    g["lift_b"] = gv.reshape(-1, s.C).sum(0)
+    return flatten(s, g)
+
+
+def loss_and_grad(y, t, w):
+    """Weighted log-space squared error, averaged over tiles:
+    L = mean_b [ sum_v w (y - t)^2 / sum_v w ];  returns (L, dL/dy)."""
+    y = np.asarray(y, np.float64)
+    t = np.asarray(t, np.float64)
+    w = np.asarray(w, np.float64)
+    B = y.shape[0]
+    ax = tuple(range(1, y.ndim))
+    sw = w.sum(axis=ax, keepdims=True)
+    d = y - t
+    L = float(np.mean((w * d * d).sum(axis=ax, keepdims=True) / sw))
+    gy = 2.0 * w * d / sw / B
+    return L, gy
