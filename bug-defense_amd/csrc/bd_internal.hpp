@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/bugdef.h"
+#include "../../include/bugdef_fno.h"
 
 namespace bd {
 

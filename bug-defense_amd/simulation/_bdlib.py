@@ -92,6 +92,10 @@ class EpochDiag(ctypes.Structure):
 _lib = None
 _lock = threading.Lock()
 
+class FnoSpec(ctypes.Structure):
+    _fields_ = [("kx", ctypes.c_int32), ("ky", ctypes.c_int32), ("kz", ctypes.c_int32), ("np", ctypes.c_int32)]
+
+
 _SIGS = {
     "bd_last_error": ([], ctypes.c_char_p),
     "bd_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -133,6 +137,27 @@ _SIGS = {
     "bd_phase_slice": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                         ctypes.c_int64], ctypes.c_int),
     "bd_sync": ([ctypes.c_void_p], ctypes.c_int),
+    # surrogate network (bugdef_fno.h); tensors are device pointers
+    "bd_fno_create": ([ctypes.c_int, ctypes.POINTER(FnoSpec), ctypes.c_int, ctypes.c_int,
+                       ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "bd_fno_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_fno_num_params": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "bd_fno_set_params": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_fno_get_params": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_fno_get_grads": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_fno_forward": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "bd_fno_backward": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "bd_fno_loss": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bd_fno_adamw": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                      ctypes.c_double], ctypes.c_int),
+    "bd_fno_sync": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_dev_alloc": ([ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "bd_dev_free": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_dev_upload": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_dev_download": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_dev_copy": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_dev_fill_f32": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_int64], ctypes.c_int),
 }
 EXPORTS = tuple(_SIGS)
 
@@ -179,6 +204,68 @@ def check(status: int, what: str = ""):
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceArray:
+    """A float32 (or other fixed-dtype) buffer in device memory, owned by the
+    library's allocator; copies are synchronous."""
+
+    def __init__(self, shape, dtype=np.float32, device=0):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().bd_dev_alloc(int(device), max(self.nbytes, 1), ctypes.byref(p)), "bd_dev_alloc")
+        self._p = p
+
+    @property
+    def ptr(self):
+        if self._p is None:
+            raise RuntimeError("device buffer is freed")
+        return self._p
+
+    def view(self, index, shape):
+        """A non-owning view of sub-array `index` along the first axis."""
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape = tuple(shape)
+        v.dtype = self.dtype
+        v.nbytes = int(np.prod(v.shape)) * self.dtype.itemsize
+        v._p = ctypes.c_void_p(self._p.value + int(index) * v.nbytes)
+        v._owner = self
+        return v
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload: {a.nbytes} bytes into a {self.nbytes}-byte buffer")
+        check(lib().bd_dev_upload(self.ptr, _ptr(a), a.nbytes), "bd_dev_upload")
+        return self
+
+    def download(self):
+        a = np.empty(self.shape, self.dtype)
+        check(lib().bd_dev_download(_ptr(a), self.ptr, a.nbytes), "bd_dev_download")
+        return a
+
+    def copy_from(self, other):
+        if other.nbytes != self.nbytes:
+            raise ValueError("copy_from: size mismatch")
+        check(lib().bd_dev_copy(self.ptr, other.ptr, self.nbytes), "bd_dev_copy")
+
+    def fill(self, value):
+        if self.dtype != np.float32:
+            raise ValueError("fill: float32 buffers only")
+        check(lib().bd_dev_fill_f32(self.ptr, float(value), self.nbytes // 4), "bd_dev_fill_f32")
+
+    def free(self):
+        if self._p is not None and getattr(self, "_owner", None) is None and self._p.value:
+            lib().bd_dev_free(self._p)
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:

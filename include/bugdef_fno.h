@@ -1,0 +1,89 @@
+/* bugdef_fno.h -- C-ABI of the absorbed-power surrogate network (gfx950).
+ *
+ * The reference has no counterpart: these entry points serve the coupled epoch
+ * loop's learned replacement of the photon-transport step.  The network maps
+ * per-voxel optical properties of a 64^3 tile (plus beam features) to the
+ * log-scaled fluence of the tile; the engine multiplies by mu_a and the
+ * incident irradiance to obtain absorbed power.
+ *
+ * Architecture: a 3-D Fourier neural operator, width 32, 4 Fourier layers,
+ * 8 input channels, projection 32 -> 64 -> 1, kept modes (kx, ky, kz) with kx
+ * and ky signed (half non-negative, half negative) and kz non-negative, over a
+ * zero-padded transform of length np >= 64.  Activations are bf16, channel
+ * mixes run on BF16 MFMA with FP32 accumulation, spectral transforms and the
+ * per-mode mixing in FP32, parameters and optimiser state in FP32.
+ *
+ * Tensors passed as *_dev are DEVICE pointers (HBM, e.g. from hipMalloc or a
+ * framework's device tensors) on the context's device; the context queues its
+ * work on its own stream, so the caller makes the inputs ready before the call
+ * and calls bd_fno_sync before reading outputs.  Layouts, channels last:
+ *   x  : float32 [B][64][64][64][8]
+ *   y  : float32 [B][64][64][64]
+ *   t,w,gy : float32 [B][64][64][64]
+ * Parameters are one flat float32 vector (order: lift W[8][32], b[32]; per
+ * layer: spectral W[kx][ky][kz][32][32][re,im], W[32][32], b[32]; projection
+ * P1[32][64], c1[64], p2[64], c2[1]).
+ *
+ * Status codes are those of bugdef.h (BD_OK, BD_EINVAL, BD_ENODEV, BD_EHIP).
+ */
+#ifndef BUGDEF_FNO_H
+#define BUGDEF_FNO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bd_fno bd_fno;
+
+typedef struct {
+    int32_t kx, ky, kz; /* kept modes: kx, ky even (<= 64), 1 <= kz <= 8 */
+    int32_t np;         /* transform length (>= 64; > 64 zero-pads the tile) */
+} bd_fno_spec;
+
+/* Create a network context on `device` for batches of up to max_batch tiles.
+ * train != 0 also allocates gradients, AdamW state and backward buffers. */
+int bd_fno_create(int device, const bd_fno_spec *spec, int max_batch, int train, bd_fno **out);
+int bd_fno_destroy(bd_fno *f);
+int bd_fno_num_params(const bd_fno *f, int64_t *n);
+
+/* Host copies of the flat parameter vector (set also clears the AdamW state)
+ * and of the last computed gradient. */
+int bd_fno_set_params(bd_fno *f, const float *host, int64_t n);
+int bd_fno_get_params(bd_fno *f, float *host, int64_t n);
+int bd_fno_get_grads(bd_fno *f, float *host, int64_t n);
+
+/* y = network(x) for B tiles (B <= max_batch). */
+int bd_fno_forward(bd_fno *f, const float *x_dev, int B, float *y_dev);
+
+/* Gradient of sum(gy * y) with respect to the parameters, for the inputs of
+ * the last bd_fno_forward (same x, same B); training contexts only. */
+int bd_fno_backward(bd_fno *f, const float *x_dev, const float *gy_dev, int B);
+
+/* Weighted squared error L = sum(w (y - t)^2) / B (w sums to one per tile)
+ * and its gradient gy = 2 w (y - t) / B; returns L (FP64, fixed-order sum). */
+int bd_fno_loss(bd_fno *f, const float *y_dev, const float *t_dev, const float *w_dev, int B, float *gy_dev,
+                double *loss);
+
+/* One AdamW update of the parameters from the current gradient (bias-corrected
+ * moments, decoupled weight decay). */
+int bd_fno_adamw(bd_fno *f, double lr, double beta1, double beta2, double eps, double weight_decay);
+
+/* Wait for the context's queued work. */
+int bd_fno_sync(bd_fno *f);
+
+/* Plain device buffers for the tensors above, so a host program needs no other
+ * GPU runtime: allocation on `device`, synchronous host<->device and
+ * device<->device copies (bytes), and a float32 fill of n values. */
+int bd_dev_alloc(int device, int64_t bytes, void **ptr);
+int bd_dev_free(void *ptr);
+int bd_dev_upload(void *dst_dev, const void *src_host, int64_t bytes);
+int bd_dev_download(void *dst_host, const void *src_dev, int64_t bytes);
+int bd_dev_copy(void *dst_dev, const void *src_dev, int64_t bytes);
+int bd_dev_fill_f32(void *dst_dev, float value, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BUGDEF_FNO_H */

@@ -35,8 +35,8 @@ from dataclasses import dataclass
 import numpy as np
 from scipy.special import erf
 
-__all__ = ["Spec", "param_shapes", "n_params", "unflatten", "tables", "forward", "backward", "gelu", "gelu_grad",
-           "init_params", "loss_and_grad"]
+__all__ = ["Spec", "param_shapes", "n_params", "unflatten", "flatten", "tables", "forward", "backward", "gelu",
+           "gelu_grad", "init_params", "loss_and_grad", "bf16"]
 
 
 @dataclass(frozen=True)
@@ -111,6 +111,17 @@ def tables(s: Spec):
     return Ex, Ey, Ez, Gz
 
 
+def bf16(a):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _ident(a):
+    return a
+
+
 def gelu(x):
     return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
 
@@ -132,21 +143,26 @@ def _spectral(s, tb, spec, v):
     return S, Xf
 
 
-def forward(s: Spec, flat, x, keep=False):
-    """x: [B, T, T, T, Cin] -> y: [B, T, T, T] (float64)."""
+def forward(s: Spec, flat, x, keep=False, emulate_bf16=False):
+    """x: [B, T, T, T, Cin] -> y: [B, T, T, T] (float64).
+
+    emulate_bf16 rounds to bfloat16 where the device does: the inputs and the
+    weights of every per-voxel channel mix (lift, W_l, P1) and each stored
+    activation v_l; everything else stays FP64."""
+    q = bf16 if emulate_bf16 else _ident
     p = unflatten(s, flat)
     tb = tables(s)
     x = np.asarray(x, dtype=np.float64)
-    v = x @ p["lift_w"] + p["lift_b"]
-    cache = {"x": x, "v": [v], "pre": [], "Xf": []}
+    v = q(q(x) @ q(p["lift_w"]) + p["lift_b"])
+    cache = {"x": x, "v": [v], "pre": [], "Xf": [], "q": q}
     for l in range(s.L):
         S, Xf = _spectral(s, tb, p[f"spec{l}"], v)
-        pre = v @ p[f"w{l}"] + p[f"b{l}"] + S
-        v = gelu(pre) if l < s.L - 1 else pre
+        pre = v @ q(p[f"w{l}"]) + p[f"b{l}"] + S
+        v = q(gelu(pre) if l < s.L - 1 else pre)
         cache["pre"].append(pre)
         cache["Xf"].append(Xf)
         cache["v"].append(v)
-    hpre = v @ p["p1"] + p["c1"]
+    hpre = v @ q(p["p1"]) + p["c1"]
     y = gelu(hpre) @ p["p2"] + p["c2"][0]
     cache["hpre"] = hpre
     return (y, cache) if keep else y
@@ -156,6 +172,7 @@ def backward(s: Spec, flat, cache, gy):
     """Gradient of sum(gy * y) with respect to the flat parameters."""
     p = unflatten(s, flat)
     Ex, Ey, Ez, Gz = tables(s)
+    q = cache.get("q", _ident)
     g = {}
     gy = np.asarray(gy, dtype=np.float64)
     hpre = cache["hpre"]
@@ -164,15 +181,15 @@ def backward(s: Spec, flat, cache, gy):
     g["c2"] = np.array([gy.sum()])
     g["p2"] = h.reshape(-1, s.H).T @ gy.reshape(-1)
     ghp = gy[..., None] * p["p2"] * gelu_grad(hpre)
-    g["p1"] = vL.reshape(-1, s.C).T @ ghp.reshape(-1, s.H)
+    g["p1"] = vL.reshape(-1, s.C).T @ q(ghp).reshape(-1, s.H)
     g["c1"] = ghp.reshape(-1, s.H).sum(0)
-    gv = ghp @ p["p1"].T
+    gv = q(ghp) @ q(p["p1"]).T
     for l in reversed(range(s.L)):
         vl, pre, Xf = cache["v"][l], cache["pre"][l], cache["Xf"][l]
         gpre = gv * gelu_grad(pre) if l < s.L - 1 else gv
-        g[f"w{l}"] = vl.reshape(-1, s.C).T @ gpre.reshape(-1, s.C)
+        g[f"w{l}"] = vl.reshape(-1, s.C).T @ q(gpre).reshape(-1, s.C)
         g[f"b{l}"] = gpre.reshape(-1, s.C).sum(0)
-        gv_new = gpre @ p[f"w{l}"].T
+        gv_new = q(gpre) @ q(p[f"w{l}"]).T
         # adjoint of S = Re(Gz U): gU = conj(Gz) gS (complex gradient convention d/dre + i d/dim)
         gU = np.einsum('kz,bxyzo->bxyko', np.conj(Gz), gpre)
         gXi = np.einsum('qy,bxyko->bxqko', Ey, gU)
@@ -186,7 +203,7 @@ def backward(s: Spec, flat, cache, gy):
         gv_spec = np.real(np.einsum('kz,bxykc->bxyzc', np.conj(Ez), gZ))
         gv = gv_new + gv_spec
     x = cache["x"]
-    g["lift_w"] = x.reshape(-1, s.Cin).T @ gv.reshape(-1, s.C)
+    g["lift_w"] = q(x).reshape(-1, s.Cin).T @ q(gv).reshape(-1, s.C)
     g["lift_b"] = gv.reshape(-1, s.C).sum(0)
     return flatten(s, g)
 

@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports every entry point include/bugdef.h declares (CPU only)."""
+"""The C-ABI library loads and exports every entry point include/*.h declares (CPU only)."""
 import ctypes
 import os
 import re
@@ -9,9 +9,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    src = open(os.path.join(ROOT, "include", "bugdef.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(bd_[a-z0-9_]+)\s*\(", src)))
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for h in sorted(os.listdir(inc)):
+        if h.endswith(".h"):
+            src = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, h)).read(), flags=re.S)
+            names |= set(re.findall(r"\b(bd_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -31,6 +35,7 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(B.Grid) == 3 * 4 + 4 + 3 * 8  # int32 x3, pad, double x3
     assert ctypes.sizeof(B.MCResult) == 7 * 8
     assert ctypes.sizeof(B.EpochDiag) == 8 + 22 * 8
+    assert ctypes.sizeof(B.FnoSpec) == 4 * 4
 
 
 def test_no_device_raises_loudly():

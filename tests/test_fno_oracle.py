@@ -118,3 +118,12 @@ def test_loss_gradient_matches_differences():
         ym[i] -= 1e-6
         fd = (F.loss_and_grad(yp, t, w)[0] - F.loss_and_grad(ym, t, w)[0]) / 2e-6
         assert gy[i] == pytest.approx(fd, rel=1e-6)
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+    rng = np.random.default_rng(7)
+    a = np.concatenate([rng.standard_normal(10000) * 10.0 ** rng.integers(-6, 6, 10000),
+                        [0.0, -0.0, 1.0, 1.00390625, 1.005859375, 65504.0]])
+    want = torch.tensor(a, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(F.bf16(a), want)
