@@ -21,7 +21,24 @@ int reduce_masked(Ctx *c, const float *src, int64_t n, const uint8_t *flags, uin
 
 using namespace bd;
 
+static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_after, bool run_mc,
+                      bd_epoch_diag *out);
+
 extern "C" int bd_epoch(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_after, bd_epoch_diag *out)
+{
+    return epoch_impl(h, beam, n_substeps, t_after, true, out);
+}
+
+// the same epoch with the absorbed-power field already in Q (a surrogate's
+// prediction): no transport, so the reflectance/transmission entries are zero
+// and the closure is not defined (reported as NaN)
+extern "C" int bd_epoch_q(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_after, bd_epoch_diag *out)
+{
+    return epoch_impl(h, beam, n_substeps, t_after, false, out);
+}
+
+static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_after, bool run_mc,
+                      bd_epoch_diag *out)
 {
     BD_CHECK(h && beam && out, "bd_epoch: null argument");
     BD_CHECK(n_substeps >= 0, "bd_epoch: n_substeps must be >= 0");
@@ -41,7 +58,13 @@ extern "C" int bd_epoch(bd_ctx *h, const bd_beam *beam, int n_substeps, double t
 
     // ---- 1. optics
     BD_HIP(hipEventRecord(e_start, c->stream));
-    if (beam->power > 0.0) {
+    if (beam->power > 0.0 && !run_mc) {
+        double qsum = 0.0;
+        BD_TRY(reduce_masked(c, c->Q, c->nvox, c->flags, F_ABL, true, &qsum));
+        out->absorbed_J = qsum * V * dt_epoch;
+        out->absorbed_frac = qsum * V / beam->power;
+        out->mc_energy_closure = std::nan("");
+    } else if (beam->power > 0.0) {
         bd_mc_result mc{};
         BD_TRY(launch_mc(c, beam, &mc));
         const double N = (double)beam->n_photons;

@@ -69,6 +69,8 @@ struct Ctx {
     double *abl_time = nullptr;
     float2 *AL = nullptr;                // interleaved absorbed / path tallies
     float4 *optics = nullptr;            // packed transport optics {mu_a, mu_s, g, n}
+    float *Qs = nullptr;                 // surrogate coupling: Q of the last MC epoch
+    float *Qsur = nullptr;               // surrogate coupling: composed field for judging
     float *P = nullptr, *P2 = nullptr, *Pext = nullptr;  // plume grid
     uint8_t *pvoid = nullptr;
     uint8_t *to_remove = nullptr;

@@ -1,0 +1,492 @@
+// bd_surrogate.hip -- coupling of the surrogate network to an engine context:
+// the 64^3 beam window, its input features, training targets from the MC
+// tallies, composition of the absorbed-power field from a prediction, the
+// judge error against an MC field, and an epoch that consumes a supplied Q.
+//
+// Window: laterally centred on the beam axis, its top a few voxels above the
+// first non-void voxel the beam axis meets (the ablation front), so the beam
+// always enters the window through void as it does in the training samples.
+//
+// Features per voxel (channels last, [64][64][64][8]):
+//   0 log1p(mu_a)   1 log1p(mu_s / 10)   2 g   3 n - 1
+//   4 ballistic: ln(max(B, 1e-6)) / ln(1e6) + 1, B = collimated beam intensity
+//     relative to the beam's peak, marched plane by plane from the window top
+//     (planar Snell invariant n sin(theta), Fresnel transmission at index
+//     steps, Beer-Lambert attenuation mu_t dz / cos(theta), lateral drift
+//     dz tan(theta) by linear interpolation)
+//   5 sin(theta_in)   6 spot / 0.5 mm - 1   7 k / 63 (depth in the window)
+// Optics are the packed transport optics the MC itself reads (plume included).
+//
+// Target: t = ln(Phi / E0 + 1e-4) with Phi = L / (N V) * P the fluence rate and
+// E0 = P / (pi w0^2 / 2) the beam's peak irradiance; weights
+// w = 0.5 Q^2 / sum Q^2 + 0.5 Phi / sum Phi (Q = mu_a Phi), summing to one.
+#include "bd_internal.hpp"
+
+#include <cmath>
+
+namespace bd {
+
+constexpr int SW = 64;                    // window edge
+constexpr float PHI_EPS = 1e-4f;          // offset of the log target
+constexpr int RAMP = 8;                   // blend width at window faces
+
+__device__ __forceinline__ float fresnel_u(float n1, float n2, float ci)
+{
+    // unpolarised Fresnel reflectance (voxel_mc_3d.py:33-48)
+    if (fabsf(n1 - n2) < 1e-9f) return 0.f;
+    ci = fminf(1.f, fabsf(ci));
+    if (ci > 0.999999f) {
+        const float r = (n1 - n2) / (n1 + n2);
+        return r * r;
+    }
+    const float e = n1 / n2;
+    const float st2 = e * e * (1.f - ci * ci);
+    if (st2 >= 1.f) return 1.f;
+    const float ct = sqrtf(1.f - st2);
+    const float rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+    const float rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+struct SurGeom {
+    const float4 *opt;
+    int Nx, Ny, Nz;
+    float dx, dy, dz;
+    int x0, y0, z0;
+};
+
+__device__ __forceinline__ bool in_dom(const SurGeom &g, int X, int Y, int Z)
+{
+    return X >= 0 && X < g.Nx && Y >= 0 && Y < g.Ny && Z >= 0 && Z < g.Nz;
+}
+
+// window origin: first non-void voxel along the beam axis, minus the margin
+__global__ void k_sur_window(const float *__restrict__ mu_a, const float *__restrict__ mu_s,
+                             const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz, float dx, float dy, float dz,
+                             float xc, float yc, float tan0, int sdir, int margin, int *__restrict__ origin)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int Y = (int)floorf(yc / dy);
+    int front = Nz;
+    for (int z = 0; z < Nz; ++z) {
+        const float xm = xc + sdir * (z + 0.5f) * dz * tan0;
+        const int X = (int)floorf(xm / dx);
+        if (X < 0 || X >= Nx || Y < 0 || Y >= Ny) continue;
+        const int64_t v = ((int64_t)X * Ny + Y) * Nz + z;
+        if ((flags[v] & F_ABL) == 0 && (mu_a[v] + mu_s[v]) > 0.f) {
+            front = z;
+            break;
+        }
+    }
+    int z0 = front - margin;
+    z0 = z0 > Nz - SW ? Nz - SW : z0;
+    z0 = z0 < 0 ? 0 : z0;
+    const float xmid = (xc + sdir * (z0 + 0.5f * SW) * dz * tan0) / dx;
+    int x0 = (int)lrintf(xmid - 0.5f * SW);
+    x0 = x0 > Nx - SW ? Nx - SW : x0;
+    x0 = x0 < 0 ? 0 : x0;
+    int y0 = (int)lrintf(yc / dy - 0.5f * SW);
+    y0 = y0 > Ny - SW ? Ny - SW : y0;
+    y0 = y0 < 0 ? 0 : y0;
+    origin[0] = x0;
+    origin[1] = y0;
+    origin[2] = z0;
+    origin[3] = front;
+}
+
+struct FeatArgs {
+    SurGeom g;
+    float sin0, cos0;
+    int sdir;
+    float xa_top, yc, w0, n_amb;
+    float f5, f6;
+    float *x;
+};
+
+// one block per window row j; threads cover x in [x0 - 64, x0 + 64) so light
+// drifting in from up-beam of the window is carried (drift < 1 voxel per plane)
+__global__ __launch_bounds__(128) void k_sur_features(FeatArgs a)
+{
+    const SurGeom &g = a.g;
+    const int j = blockIdx.x, t = threadIdx.x;
+    const int X = g.x0 - SW + t, Y = g.y0 + j;
+    __shared__ float sB[128];
+    const float xm = (X + 0.5f) * g.dx, ym = (Y + 0.5f) * g.dy;
+    const float u = (xm - a.xa_top) * a.cos0, v = ym - a.yc;
+    float B = expf(-2.f * (u * u + v * v) / (a.w0 * a.w0));
+    float nprev = a.n_amb;
+    const float s0 = a.n_amb * a.sin0;
+    for (int k = 0; k < SW; ++k) {
+        const int Z = g.z0 + k;
+        float4 o = make_float4(0.f, 0.f, 0.f, a.n_amb);
+        if (in_dom(g, X, Y, Z)) o = g.opt[((int64_t)X * g.Ny + Y) * g.Nz + Z];
+        sB[t] = B;
+        __syncthreads();
+        const int tn = t - a.sdir;
+        const float Bn = (tn >= 0 && tn < 128) ? sB[tn] : B;
+        __syncthreads();
+        const float sk = s0 / o.w;
+        float Bk = 0.f;
+        if (sk < 1.f) {
+            const float ck = sqrtf(1.f - sk * sk);
+            const float sh = fminf(g.dz * sk / (ck * g.dx), 1.f);
+            const float bi = (1.f - sh) * B + sh * Bn;
+            const float sp = s0 / nprev;
+            const float cp = sqrtf(fmaxf(0.f, 1.f - sp * sp));
+            const float R = fresnel_u(nprev, o.w, cp);
+            Bk = bi * (1.f - R) * expf(-(o.x + o.y) * g.dz / ck);
+        }
+        B = Bk;
+        nprev = o.w;
+        if (t >= SW) {
+            float *xp = a.x + (((int64_t)(t - SW) * SW + j) * SW + k) * 8;
+            xp[0] = log1pf(o.x);
+            xp[1] = log1pf(o.y * 0.1f);
+            xp[2] = o.z;
+            xp[3] = o.w - 1.f;
+            xp[4] = logf(fmaxf(B, 1e-6f)) * (1.f / 13.815510557964274f) + 1.f;
+            xp[5] = a.f5;
+            xp[6] = a.f6;
+            xp[7] = k * (1.f / 63.f);
+        }
+    }
+}
+
+// per-plane sums of Q^2 and Phi over the window (FP64), one block per plane i
+__global__ __launch_bounds__(256) void k_sur_tsum(SurGeom g, const float2 *__restrict__ AL, float scale,
+                                                  double *__restrict__ part)
+{
+    __shared__ double sq[256], sp[256];
+    const int i = blockIdx.x;
+    double q2 = 0.0, ph = 0.0;
+    for (int idx = threadIdx.x; idx < SW * SW; idx += blockDim.x) {
+        const int j = idx / SW, k = idx % SW;
+        const int X = g.x0 + i, Y = g.y0 + j, Z = g.z0 + k;
+        if (!in_dom(g, X, Y, Z)) continue;
+        const int64_t v = ((int64_t)X * g.Ny + Y) * g.Nz + Z;
+        const float phi = AL[v].y * scale;
+        const float q = g.opt[v].x * phi;
+        q2 += (double)q * q;
+        ph += phi;
+    }
+    sq[threadIdx.x] = q2;
+    sp[threadIdx.x] = ph;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sq[threadIdx.x] += sq[threadIdx.x + h];
+            sp[threadIdx.x] += sp[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * i] = sq[0];
+        part[2 * i + 1] = sp[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sur_twrite(SurGeom g, const float2 *__restrict__ AL, float scale,
+                                                    const double *__restrict__ part, float *__restrict__ tg,
+                                                    float *__restrict__ wt)
+{
+    double tq = 0.0, tp = 0.0;
+    for (int i = 0; i < SW; ++i) {
+        tq += part[2 * i];
+        tp += part[2 * i + 1];
+    }
+    const float iq = tq > 0.0 ? (float)(0.5 / tq) : 0.f, ip = tp > 0.0 ? (float)(0.5 / tp) : 0.f;
+    const float uni = 1.f / (float)(SW * SW * SW);
+    const int i = blockIdx.x;
+    for (int idx = threadIdx.x; idx < SW * SW; idx += blockDim.x) {
+        const int j = idx / SW, k = idx % SW;
+        const int X = g.x0 + i, Y = g.y0 + j, Z = g.z0 + k;
+        float phi = 0.f, q = 0.f;
+        if (in_dom(g, X, Y, Z)) {
+            const int64_t v = ((int64_t)X * g.Ny + Y) * g.Nz + Z;
+            phi = AL[v].y * scale;
+            q = g.opt[v].x * phi;
+        }
+        const int64_t o = ((int64_t)i * SW + j) * SW + k;
+        tg[o] = logf(phi + PHI_EPS);
+        wt[o] = (tq > 0.0 ? q * q * iq : 0.5f * uni) + (tp > 0.0 ? phi * ip : 0.5f * uni);
+    }
+}
+
+__device__ __forceinline__ float ramp(int i, bool lo, bool hi)
+{
+    float r = 1.f;
+    if (lo) r = fminf(r, (i + 0.5f) / RAMP);
+    if (hi) r = fminf(r, (SW - i - 0.5f) / RAMP);
+    return r;
+}
+
+// Q over the window from the prediction, blended into the stale field at the
+// window faces that lie inside the domain
+__global__ void k_sur_apply(SurGeom g, const float *__restrict__ y, const float *__restrict__ Qs,
+                            float *__restrict__ Qo, float E0, int blend_mask)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)SW * SW * SW;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t % SW), j = (int)((t / SW) % SW), i = (int)(t / (SW * SW));
+        const int X = g.x0 + i, Y = g.y0 + j, Z = g.z0 + k;
+        if (!in_dom(g, X, Y, Z)) continue;
+        const int64_t v = ((int64_t)X * g.Ny + Y) * g.Nz + Z;
+        const float qs = E0 * g.opt[v].x * fmaxf(expf(y[t]) - PHI_EPS, 0.f);
+        const float r = ramp(i, blend_mask & 1, blend_mask & 2) * ramp(j, blend_mask & 4, blend_mask & 8) *
+                        ramp(k, blend_mask & 16, blend_mask & 32);
+        Qo[v] = r * qs + (1.f - r) * Qs[v];
+    }
+}
+
+__global__ void k_copy_f32(const float *__restrict__ s, float *__restrict__ d, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+// judge error: per-block partials of sum (a - b)^2, sum a^2 over non-ablated
+// voxels, and the same over 4^3 block sums (MC shot noise averaged down)
+__global__ __launch_bounds__(256) void k_sur_err(const float *__restrict__ a, const float *__restrict__ b,
+                                                 const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz,
+                                                 double *__restrict__ part)
+{
+    __shared__ double s[4][256];
+    double d2 = 0.0, a2 = 0.0, bd2 = 0.0, ba2 = 0.0;
+    const int bx = (Nx + 3) / 4, by = (Ny + 3) / 4, bz = (Nz + 3) / 4;
+    const int64_t nb = (int64_t)bx * by * bz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nb; t += (int64_t)gridDim.x * blockDim.x) {
+        const int kz = (int)(t % bz), jy = (int)((t / bz) % by), ix = (int)(t / ((int64_t)bz * by));
+        double sa = 0.0, sb = 0.0;
+        for (int i = 4 * ix; i < 4 * ix + 4 && i < Nx; ++i)
+            for (int j = 4 * jy; j < 4 * jy + 4 && j < Ny; ++j)
+                for (int k = 4 * kz; k < 4 * kz + 4 && k < Nz; ++k) {
+                    const int64_t v = ((int64_t)i * Ny + j) * Nz + k;
+                    if (flags[v] & F_ABL) continue;
+                    const double x = a[v], w = b[v];
+                    d2 += (x - w) * (x - w);
+                    a2 += x * x;
+                    sa += x;
+                    sb += w;
+                }
+        bd2 += (sa - sb) * (sa - sb);
+        ba2 += sa * sa;
+    }
+    s[0][threadIdx.x] = d2;
+    s[1][threadIdx.x] = a2;
+    s[2][threadIdx.x] = bd2;
+    s[3][threadIdx.x] = ba2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) part[4 * blockIdx.x + q] = s[q][0];
+}
+
+__global__ void k_sur_err_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < nb; ++b)
+        for (int q = 0; q < 4; ++q) r[q] += part[4 * b + q];
+    for (int q = 0; q < 4; ++q) out[q] = r[q];
+}
+
+namespace {
+
+struct BeamFrame {
+    float sin0, cos0, tan0;
+    int sdir;
+    float w0;
+};
+
+int beam_frame(const bd_beam *b, BeamFrame &f)
+{
+    const double c = std::cos(b->phi_rad), s = std::sin(b->phi_rad);
+    if (b->launch_axis != 0 || std::fabs(s) > 1e-9) {
+        set_error("surrogate: only a beam launched along +z with azimuth 0 or pi is supported");
+        return BD_EINVAL;
+    }
+    if (!(b->theta_rad >= 0.0 && b->theta_rad < 1.2) || !(b->spot_dia > 0.0)) {
+        set_error("surrogate: beam angle must be in [0, 1.2) rad and the spot > 0");
+        return BD_EINVAL;
+    }
+    f.sin0 = (float)std::sin(b->theta_rad);
+    f.cos0 = (float)std::cos(b->theta_rad);
+    f.tan0 = (float)std::tan(b->theta_rad);
+    f.sdir = c >= 0.0 ? 1 : -1;
+    f.w0 = (float)(b->spot_dia / 2.0);
+    return BD_OK;
+}
+
+SurGeom geom(Ctx *c, const int32_t *o)
+{
+    SurGeom g{};
+    g.opt = c->optics;
+    g.Nx = c->g.nx;
+    g.Ny = c->g.ny;
+    g.Nz = c->g.nz;
+    g.dx = (float)c->g.dx;
+    g.dy = (float)c->g.dy;
+    g.dz = (float)c->g.dz;
+    g.x0 = o[0];
+    g.y0 = o[1];
+    g.z0 = o[2];
+    return g;
+}
+
+int check_origin(Ctx *c, const int32_t *o)
+{
+    if (!o) {
+        set_error("surrogate: null window origin");
+        return BD_EINVAL;
+    }
+    const int n[3] = {c->g.nx, c->g.ny, c->g.nz};
+    for (int d = 0; d < 3; ++d)
+        if (o[d] < 0 || (n[d] >= SW && o[d] > n[d] - SW) || (n[d] < SW && o[d] != 0)) {
+            set_error("surrogate: window origin outside the domain");
+            return BD_EINVAL;
+        }
+    return BD_OK;
+}
+
+}  // namespace
+}  // namespace bd
+
+using namespace bd;
+
+extern "C" {
+
+int bd_sur_window(bd_ctx *h, const bd_beam *beam, int margin, int32_t *origin)
+{
+    BD_CHECK(h && beam && origin, "bd_sur_window: null argument");
+    BD_CHECK(margin >= 0 && margin < SW, "bd_sur_window: margin must be in [0, 64)");
+    Ctx *c = &h->c;
+    BD_TRY(ensure_state(c));
+    BD_CHECK(c->mu_a && c->mu_s, "bd_sur_window: optics fields are not uploaded");
+    BeamFrame f;
+    BD_TRY(beam_frame(beam, f));
+    int *dev = (int *)(c->counter + 8);
+    hipLaunchKernelGGL(k_sur_window, dim3(1), dim3(64), 0, c->stream, c->mu_a, c->mu_s, c->flags, c->g.nx, c->g.ny,
+                       c->g.nz, (float)c->g.dx, (float)c->g.dy, (float)c->g.dz, (float)beam->x_center,
+                       (float)beam->y_center, f.tan0, f.sdir, margin, dev);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipMemcpyAsync(c->host_pinned, dev, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    const int *r = (const int *)c->host_pinned;
+    for (int d = 0; d < 4; ++d) origin[d] = r[d];
+    return BD_OK;
+}
+
+int bd_sur_features(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float *x_dev)
+{
+    BD_CHECK(h && beam && x_dev, "bd_sur_features: null argument");
+    Ctx *c = &h->c;
+    BD_TRY(ensure_state(c));
+    BD_TRY(check_origin(c, origin));
+    BeamFrame f;
+    BD_TRY(beam_frame(beam, f));
+    BD_TRY(launch_pack_optics(c, beam->add_plume != 0));
+    FeatArgs a{};
+    a.g = geom(c, origin);
+    a.sin0 = f.sin0;
+    a.cos0 = f.cos0;
+    a.sdir = f.sdir;
+    a.xa_top = (float)(beam->x_center + f.sdir * origin[2] * c->g.dz * f.tan0);
+    a.yc = (float)beam->y_center;
+    a.w0 = f.w0;
+    a.n_amb = (float)beam->n_ambient;
+    a.f5 = f.sin0;
+    a.f6 = (float)(beam->spot_dia / 0.5 - 1.0);
+    a.x = x_dev;
+    hipLaunchKernelGGL(k_sur_features, dim3(SW), dim3(128), 0, c->stream, a);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int bd_sur_targets(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float *t_dev, float *w_dev)
+{
+    BD_CHECK(h && beam && t_dev && w_dev, "bd_sur_targets: null argument");
+    Ctx *c = &h->c;
+    BD_TRY(check_origin(c, origin));
+    BD_CHECK(c->AL && c->optics, "bd_sur_targets: no MC tallies on the context (run bd_mc_run first)");
+    BD_CHECK(beam->n_photons > 0, "bd_sur_targets: n_photons must be > 0");
+    BeamFrame f;
+    BD_TRY(beam_frame(beam, f));
+    const double V = c->g.dx * c->g.dy * c->g.dz;
+    const float scale = (float)(M_PI * (double)f.w0 * f.w0 / 2.0 / ((double)beam->n_photons * V));
+    const SurGeom g = geom(c, origin);
+    double *part = c->red + 64;
+    hipLaunchKernelGGL(k_sur_tsum, dim3(SW), dim3(256), 0, c->stream, g, c->AL, scale, part);
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sur_twrite, dim3(SW), dim3(256), 0, c->stream, g, c->AL, scale, part, t_dev, w_dev);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int bd_sur_apply(bd_ctx *h, const bd_beam *beam, const int32_t *origin, const float *y_dev, int into_q)
+{
+    BD_CHECK(h && beam && y_dev, "bd_sur_apply: null argument");
+    Ctx *c = &h->c;
+    BD_TRY(ensure_state(c));
+    BD_TRY(check_origin(c, origin));
+    BD_CHECK(c->optics, "bd_sur_apply: optics are not packed (call bd_sur_features first)");
+    BeamFrame f;
+    BD_TRY(beam_frame(beam, f));
+    BD_TRY(ensure(c, (void **)&c->Qs, (size_t)c->nvox * 4));
+    float *dst = c->Q;
+    if (!into_q) {
+        BD_TRY(ensure(c, (void **)&c->Qsur, (size_t)c->nvox * 4));
+        dst = c->Qsur;
+    }
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(256), 0, c->stream, c->Qs, dst, c->nvox);
+    BD_HIP(hipGetLastError());
+    const float E0 = (float)(beam->power / (M_PI * (double)f.w0 * f.w0 / 2.0));
+    int mask = 0;
+    const int n[3] = {c->g.nx, c->g.ny, c->g.nz};
+    for (int d = 0; d < 3; ++d) {
+        if (origin[d] > 0) mask |= 1 << (2 * d);
+        if (origin[d] + SW < n[d]) mask |= 2 << (2 * d);
+    }
+    hipLaunchKernelGGL(k_sur_apply, dim3(1024), dim3(256), 0, c->stream, geom(c, origin), y_dev, c->Qs, dst, E0,
+                       mask);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int bd_sur_keep_stale(bd_ctx *h)
+{
+    BD_CHECK(h, "bd_sur_keep_stale: null argument");
+    Ctx *c = &h->c;
+    BD_TRY(ensure_state(c));
+    BD_TRY(ensure(c, (void **)&c->Qs, (size_t)c->nvox * 4));
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(256), 0, c->stream, c->Q, c->Qs, c->nvox);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int bd_sur_error(bd_ctx *h, double *out)
+{
+    BD_CHECK(h && out, "bd_sur_error: null argument");
+    Ctx *c = &h->c;
+    BD_CHECK(c->Qsur && c->Q, "bd_sur_error: no surrogate field to compare (bd_sur_apply with into_q = 0)");
+    constexpr int NB = 256;
+    double *part = c->red + 64;
+    hipLaunchKernelGGL(k_sur_err, dim3(NB), dim3(256), 0, c->stream, c->Q, c->Qsur, c->flags, c->g.nx, c->g.ny,
+                       c->g.nz, part);
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sur_err_final, dim3(1), dim3(64), 0, c->stream, part, NB, c->red + 48);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 48, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    const double *r = (const double *)c->host_pinned;
+    out[0] = r[1] > 0.0 ? std::sqrt(r[0] / r[1]) : 0.0;
+    out[1] = r[3] > 0.0 ? std::sqrt(r[2] / r[3]) : 0.0;
+    return BD_OK;
+}
+
+}  // extern "C"

@@ -137,6 +137,19 @@ _SIGS = {
     "bd_phase_slice": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                         ctypes.c_int64], ctypes.c_int),
     "bd_sync": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_epoch_q": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double,
+                    ctypes.POINTER(EpochDiag)], ctypes.c_int),
+    # surrogate coupling (bugdef_fno.h)
+    "bd_sur_window": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32)],
+                      ctypes.c_int),
+    "bd_sur_features": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p],
+                        ctypes.c_int),
+    "bd_sur_targets": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p,
+                        ctypes.c_void_p], ctypes.c_int),
+    "bd_sur_apply": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p,
+                      ctypes.c_int], ctypes.c_int),
+    "bd_sur_keep_stale": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_sur_error": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     # surrogate network (bugdef_fno.h); tensors are device pointers
     "bd_fno_create": ([ctypes.c_int, ctypes.POINTER(FnoSpec), ctypes.c_int, ctypes.c_int,
                        ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
@@ -428,6 +441,42 @@ class Context:
         check(lib().bd_epoch_graph(self.handle, ctypes.byref(beam), int(n_substeps), float(t_before), int(epoch),
                                    int(seed_base) & 0xFFFFFFFFFFFFFFFF, ctypes.byref(d)), "bd_epoch_graph")
         return d
+
+    def epoch_q(self, beam: Beam, n_substeps, t_after) -> EpochDiag:
+        d = EpochDiag()
+        check(lib().bd_epoch_q(self.handle, ctypes.byref(beam), int(n_substeps), float(t_after), ctypes.byref(d)),
+              "bd_epoch_q")
+        return d
+
+    # -- surrogate coupling (include/bugdef_fno.h) ------------------------------
+    def sur_window(self, beam: Beam, margin=12):
+        o = (ctypes.c_int32 * 4)()
+        check(lib().bd_sur_window(self.handle, ctypes.byref(beam), int(margin), o), "bd_sur_window")
+        return tuple(int(v) for v in o)
+
+    @staticmethod
+    def _origin(origin):
+        return (ctypes.c_int32 * 3)(*(int(v) for v in origin[:3]))
+
+    def sur_features(self, beam: Beam, origin, x_dev):
+        check(lib().bd_sur_features(self.handle, ctypes.byref(beam), self._origin(origin), x_dev.ptr),
+              "bd_sur_features")
+
+    def sur_targets(self, beam: Beam, origin, t_dev, w_dev):
+        check(lib().bd_sur_targets(self.handle, ctypes.byref(beam), self._origin(origin), t_dev.ptr, w_dev.ptr),
+              "bd_sur_targets")
+
+    def sur_apply(self, beam: Beam, origin, y_dev, into_q):
+        check(lib().bd_sur_apply(self.handle, ctypes.byref(beam), self._origin(origin), y_dev.ptr,
+                                 1 if into_q else 0), "bd_sur_apply")
+
+    def sur_keep_stale(self):
+        check(lib().bd_sur_keep_stale(self.handle), "bd_sur_keep_stale")
+
+    def sur_error(self):
+        out = (ctypes.c_double * 2)()
+        check(lib().bd_sur_error(self.handle, out), "bd_sur_error")
+        return float(out[0]), float(out[1])
 
     def reduce(self, name, op="sum", mask=None):
         ops = {"sum": 0, "max": 1, "min": 2}

@@ -423,8 +423,8 @@ class CoupledAblationEngine:
                     self._ctx.upload(k.upper(), opt[k])
 
     # ---------------------------------------------------------------- epoch
-    def step_epoch(self) -> dict:
-        """Advance one coupled epoch (coupled_ablation.py:440-617); returns its diagnostics."""
+    def _epoch_beam(self):
+        """Flush host-side writes, push the physics, and form this epoch's beam."""
         self._flush_host_writes()
         if self._ds_arrays is None and self.dry_state is not None:
             self._init_ds_cs_arrays()
@@ -433,25 +433,16 @@ class CoupledAblationEngine:
         la_centre = [self.Lx / 2.0, self.Ly / 2.0, 0.0]
         if self.beam_center is not None:
             la_centre[0], la_centre[1] = float(self.beam_center[0]), float(self.beam_center[1])
-        beam = make_beam(self.power, self.n_photons, self.spot_dia, self.theta_inc_deg, self.phi_inc_deg, la_centre,
+        return make_beam(self.power, self.n_photons, self.spot_dia, self.theta_inc_deg, self.phi_inc_deg, la_centre,
                          self.r_max, 0, self.beam_profile, mix_seed(self.seed, self.epoch),
                          score_fluence=self.score_fluence, add_plume=self.plume is not None)
-        t_after = self.t + self.dt_epoch
-        t0 = time.perf_counter()
-        graphable = (self.graph and self.n_substeps % 2 == 0 and abs(self.n_substeps * self.dt_sub - self.dt_epoch)
-                     <= 1e-12 * self.dt_epoch and (self.plume is None or (self.plume.model == 'diffusion'
-                                                                          and not self.plume.blast_expansion)))
-        if graphable:
-            # same MC key (mix_seed(seed, epoch)) and removal time (t + dt_epoch), formed on the device
-            d = self._ctx.epoch_graph(beam, self.n_substeps, self.t, self.epoch, self.seed)
-        else:
-            d = self._ctx.epoch(beam, self.n_substeps, t_after)
-        wall = time.perf_counter() - t0
+
+    def _finish_epoch(self, d, t_after, wall, mc=True) -> dict:
         self._invalidate()
         self.t = t_after
         self.epoch += 1
         self.last_diag = d
-        if self.power > 0.0:
+        if self.power > 0.0 and mc:
             self.last_mc = _LazyMC(self, d)
         return {
             'epoch': self.epoch,
@@ -470,6 +461,32 @@ class CoupledAblationEngine:
             'ledger_J': {'absorbed': d.absorbed_J, 'reflected': d.reflected_J, 'transmitted': d.transmitted_J,
                          'lateral': d.lateral_J, 'conducted': d.conducted_J, 'ablated': d.ablated_J},
         }
+
+    def step_epoch(self) -> dict:
+        """Advance one coupled epoch (coupled_ablation.py:440-617); returns its diagnostics."""
+        beam = self._epoch_beam()
+        t_after = self.t + self.dt_epoch
+        t0 = time.perf_counter()
+        graphable = (self.graph and self.n_substeps % 2 == 0 and abs(self.n_substeps * self.dt_sub - self.dt_epoch)
+                     <= 1e-12 * self.dt_epoch and (self.plume is None or (self.plume.model == 'diffusion'
+                                                                          and not self.plume.blast_expansion)))
+        if graphable:
+            # same MC key (mix_seed(seed, epoch)) and removal time (t + dt_epoch), formed on the device
+            d = self._ctx.epoch_graph(beam, self.n_substeps, self.t, self.epoch, self.seed)
+        else:
+            d = self._ctx.epoch(beam, self.n_substeps, t_after)
+        return self._finish_epoch(d, t_after, time.perf_counter() - t0)
+
+    def step_epoch_with_q(self) -> dict:
+        """One epoch whose absorbed-power field is already on the device (written
+        by a surrogate into Q): sub-steps, phase update, removal and plume as in
+        step_epoch, no transport.  ``mc_energy_closure`` is NaN and the
+        reflected / transmitted / lateral ledger entries are zero."""
+        beam = self._epoch_beam()
+        t_after = self.t + self.dt_epoch
+        t0 = time.perf_counter()
+        d = self._ctx.epoch_q(beam, self.n_substeps, t_after)
+        return self._finish_epoch(d, t_after, time.perf_counter() - t0, mc=False)
 
     def compute_void_mask(self) -> np.ndarray:
         """coupled_ablation.py:619-626."""

@@ -1,0 +1,86 @@
+"""The coupled epoch loop with the surrogate in it (judge / fallback).
+
+``SurrogateCoupledEngine`` drives a ``CoupledAblationEngine``: on most epochs
+the network supplies the absorbed power over the 64^3 beam window (features
+from the current optics, one forward pass, Q composed with the field of the
+last MC epoch outside the window) and the epoch runs without transport
+(``step_epoch_with_q``); every ``judge_every``-th epoch the plain MC epoch runs
+and judges the network, whose prediction for the same state is composed into a
+separate field and compared with the MC field.  When the judged relative L2
+error exceeds ``tol``, the loop falls back to MC on every epoch (still judging
+each one) until the error is back within ``tol``.
+
+The judged error is the relative L2 over 4x4x4 block sums of the absorbed
+power (non-ablated voxels), which averages down the MC's own shot noise; the
+per-voxel figure is recorded beside it.
+"""
+
+import time
+
+from simulation._bdlib import DeviceArray
+
+from .network import CIN, T, FourierSurrogate
+
+
+class SurrogateCoupledEngine:
+    def __init__(self, engine, weights, judge_every=10, tol=0.05, margin=12, metric="block4"):
+        if engine.plume is not None and engine.plume.model != 'diffusion':
+            raise ValueError("the surrogate loop supports the diffusion plume only")
+        self.engine = engine
+        self.net = weights if isinstance(weights, FourierSurrogate) else FourierSurrogate.load(weights, max_batch=1)
+        self.judge_every = int(judge_every)
+        self.tol = float(tol)
+        self.margin = int(margin)
+        self.metric = metric
+        self.x = DeviceArray((1, T, T, T, CIN))
+        self.y = DeviceArray((1, T, T, T))
+        self.fallback = False
+        self.have_stale = False
+        self.history = []
+
+    def _predict(self, beam, into_q):
+        ctx = self.engine._ctx
+        origin = ctx.sur_window(beam, self.margin)
+        ctx.sur_features(beam, origin, self.x.view(0, (T, T, T, CIN)))
+        ctx.sync()
+        self.net.forward(self.x, self.y)
+        ctx.sur_apply(beam, origin, self.y.view(0, (T, T, T)), into_q)
+        return origin
+
+    def step_epoch(self) -> dict:
+        eng = self.engine
+        if eng.power <= 0.0:
+            d = eng.step_epoch()
+            d['optics_source'] = 'none'
+            return d
+        judge = (not self.have_stale) or self.fallback or (eng.epoch % self.judge_every == 0)
+        t0 = time.perf_counter()
+        if judge:
+            err = None
+            if self.have_stale:
+                beam = eng._epoch_beam()
+                self._predict(beam, into_q=False)
+            d = eng.step_epoch()
+            if self.have_stale:
+                err = eng._ctx.sur_error()
+                e = err[1] if self.metric == "block4" else err[0]
+                self.fallback = e > self.tol
+            eng._ctx.sur_keep_stale()
+            self.have_stale = True
+            d['optics_source'] = 'mc'
+            d['judge_error'] = err
+        else:
+            beam = eng._epoch_beam()
+            self._predict(beam, into_q=True)
+            d = eng.step_epoch_with_q()
+            d['optics_source'] = 'surrogate'
+            d['judge_error'] = None
+        d['t_wall_s'] = time.perf_counter() - t0
+        d['fallback'] = self.fallback
+        self.history.append({k: d[k] for k in ('epoch', 'optics_source', 'judge_error', 'fallback', 'n_new')})
+        return d
+
+    def close(self):
+        self.x.free()
+        self.y.free()
+        self.net.close()

@@ -22,6 +22,31 @@ DEFAULT_MODES = (8, 8, 4)
 DEFAULT_NP = 80
 
 
+def param_layout(modes=DEFAULT_MODES):
+    """Ordered (name, shape) of the flat parameter vector (include/bugdef_fno.h)."""
+    kx, ky, kz = modes
+    out = [("lift_w", (CIN, C)), ("lift_b", (C,))]
+    for l in range(L):
+        out += [(f"spec{l}", (kx, ky, kz, C, C, 2)), (f"w{l}", (C, C)), (f"b{l}", (C,))]
+    out += [("p1", (C, H)), ("c1", (H,)), ("p2", (H,)), ("c2", (1,))]
+    return out
+
+
+def init_params(modes=DEFAULT_MODES, seed=0, spectral_scale=2.0):
+    """Fan-in scaled normal weights, zero biases; spectral weights
+    spectral_scale / (2 C) per real component."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, sh in param_layout(modes):
+        if name.startswith("spec"):
+            parts.append(rng.standard_normal(sh) * spectral_scale / (2.0 * C))
+        elif name in ("lift_w", "p1", "p2") or (name[0] == "w" and name[1:].isdigit()):
+            parts.append(rng.standard_normal(sh) / np.sqrt(sh[0]))
+        else:
+            parts.append(np.zeros(sh))
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
 def _dev_ptr(t, shape, what):
     if not (isinstance(t, B.DeviceArray) and t.dtype == np.float32):
         raise ValueError(f"{what}: expected a float32 DeviceArray")
@@ -49,6 +74,18 @@ class FourierSurrogate:
         self.n_params = int(n.value)
 
     # -- parameters -----------------------------------------------------------
+    def save(self, path, **meta):
+        np.savez(path, params=self.get_params(), modes=np.array(self.modes), np_len=self.np_len,
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path, max_batch=8, train=False, device=0):
+        with np.load(path, allow_pickle=False) as z:
+            net = cls(modes=tuple(int(v) for v in z["modes"]), np_len=int(z["np_len"]), max_batch=max_batch,
+                      train=train, device=device)
+            net.set_params(z["params"])
+        return net
+
     def set_params(self, flat):
         a = np.ascontiguousarray(flat, dtype=np.float32)
         if a.size != self.n_params:

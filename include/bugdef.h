@@ -237,6 +237,12 @@ int bd_phase_map(bd_ctx *ctx, const uint8_t *is_air_gap, uint8_t *out, int64_t n
  * without moving the 3-D grids.  out is uint8, n values. */
 int bd_phase_slice(bd_ctx *ctx, int plane, int index, int stride, const uint8_t *gap_z, uint8_t *out, int64_t n);
 
+/* bd_epoch with the absorbed-power field already in Q instead of a transport
+ * step (coupled_ablation.py:502-617 from the sub-steps on): used when a
+ * surrogate supplies Q.  absorbed_J / absorbed_frac come from Q; reflected,
+ * transmitted and lateral are zero and mc_energy_closure is NaN. */
+int bd_epoch_q(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_after, bd_epoch_diag *out);
+
 /* Wait for all queued work of the context. */
 int bd_sync(bd_ctx *ctx);
 

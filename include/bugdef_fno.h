@@ -31,6 +31,8 @@
 
 #include <stdint.h>
 
+#include "bugdef.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -72,6 +74,39 @@ int bd_fno_adamw(bd_fno *f, double lr, double beta1, double beta2, double eps, d
 
 /* Wait for the context's queued work. */
 int bd_fno_sync(bd_fno *f);
+
+/* ---- coupling to an engine context (bugdef.h) -------------------------------
+ * The network sees a 64^3 window of the domain.  Supported beams: launched
+ * along +z with azimuth 0 or pi (the drift of the collimated beam stays in
+ * the x-z plane); others are rejected with BD_EINVAL.
+ *
+ * Window origin (voxel indices, written to origin[0..2]) laterally centred on
+ * the beam axis with its top `margin` voxels above the first non-void voxel
+ * on the axis; origin[3] receives that voxel's z index (Nz if none). */
+int bd_sur_window(bd_ctx *ctx, const bd_beam *beam, int margin, int32_t *origin);
+
+/* Input features of the window (x_dev: float32 [64][64][64][8], channels
+ * log1p(mu_a), log1p(mu_s/10), g, n-1, ballistic beam log-intensity,
+ * sin(theta), spot/0.5mm - 1, depth/63) from the packed transport optics. */
+int bd_sur_features(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, float *x_dev);
+
+/* Training target t = ln(Phi/E0 + 1e-4) and loss weights (sum one) of the
+ * window from the context's last MC tallies (bd_mc_run with score_fluence). */
+int bd_sur_targets(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, float *t_dev, float *w_dev);
+
+/* Compose the absorbed power from a prediction y_dev ([64][64][64]):
+ * E0 mu_a (exp(y) - 1e-4)+ inside the window, blended over 8 voxels into the
+ * field kept by bd_sur_keep_stale at faces inside the domain, that field
+ * elsewhere.  into_q != 0 writes the context's Q (then run bd_epoch_q);
+ * into_q == 0 writes a separate field that bd_sur_error compares with Q. */
+int bd_sur_apply(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, const float *y_dev, int into_q);
+
+/* Keep the current Q (an MC epoch's field) as the field outside the window. */
+int bd_sur_keep_stale(bd_ctx *ctx);
+
+/* Relative L2 error of the composed field against Q over non-ablated voxels:
+ * out[0] per voxel, out[1] over 4x4x4 block sums (MC shot noise averaged). */
+int bd_sur_error(bd_ctx *ctx, double *out);
 
 /* Plain device buffers for the tensors above, so a host program needs no other
  * GPU runtime: allocation on `device`, synchronous host<->device and

@@ -1,0 +1,206 @@
+"""FP64 NumPy restatement of the surrogate coupling (window, features, targets,
+composition, judge error) -- test infrastructure only.
+
+Only the tests, the smoke check and the benchmark's CPU baseline may import this
+module, and only as the checker.  The device versions live in
+``bug-defense_amd/csrc/bd_surrogate.hip``; the reference has no learned
+component, so these functions define the coupling rather than restate the
+reference.  Pins that do not depend on these formulas (``test_surrogate_oracle.py``):
+Beer-Lambert decay with the normal-incidence Fresnel factor in a homogeneous
+slab, the Snell drift dz tan(theta_t) of a narrow oblique beam's centroid,
+weights summing to one, and exp(target) - 1e-4 recovering Phi / E0.
+"""
+
+import math
+
+import numpy as np
+
+SW = 64
+PHI_EPS = 1e-4
+RAMP = 8
+LN1E6 = math.log(1e6)
+
+
+def fresnel_u(n1, n2, ci):
+    """Unpolarised Fresnel reflectance (voxel_mc_3d.py:33-48), vectorised."""
+    n1 = np.broadcast_to(np.asarray(n1, np.float64), np.shape(ci)).astype(np.float64)
+    n2 = np.broadcast_to(np.asarray(n2, np.float64), np.shape(ci)).astype(np.float64)
+    ci = np.minimum(1.0, np.abs(np.asarray(ci, np.float64)))
+    out = np.zeros(np.shape(ci))
+    same = np.abs(n1 - n2) < 1e-9
+    normal = (~same) & (ci > 0.999999)
+    r = (n1 - n2) / (n1 + n2)
+    out[normal] = (r * r)[normal]
+    rest = (~same) & (~normal)
+    e = n1 / n2
+    st2 = e * e * (1.0 - ci * ci)
+    tir = rest & (st2 >= 1.0)
+    out[tir] = 1.0
+    ok = rest & (st2 < 1.0)
+    ct = np.sqrt(np.clip(1.0 - st2, 0.0, None))
+    rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct)
+    rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci)
+    out[ok] = (0.5 * (rs * rs + rp * rp))[ok]
+    return out
+
+
+def window(mu_a, mu_s, ablated, spacing, xc, yc, theta, phi=0.0, margin=12):
+    """Window origin and the first non-void z on the beam axis."""
+    Nx, Ny, Nz = mu_a.shape
+    dx, dy, dz = spacing
+    sdir = 1 if math.cos(phi) >= 0 else -1
+    tan0 = math.tan(theta)
+    Y = int(math.floor(yc / dy))
+    front = Nz
+    for z in range(Nz):
+        X = int(math.floor((xc + sdir * (z + 0.5) * dz * tan0) / dx))
+        if not (0 <= X < Nx and 0 <= Y < Ny):
+            continue
+        if not ablated[X, Y, z] and mu_a[X, Y, z] + mu_s[X, Y, z] > 0:
+            front = z
+            break
+    z0 = min(front - margin, Nz - SW)
+    z0 = max(z0, 0)
+    xmid = (xc + sdir * (z0 + 0.5 * SW) * dz * tan0) / dx
+    x0 = max(min(int(round(xmid - 0.5 * SW)), Nx - SW), 0)
+    y0 = max(min(int(round(yc / dy - 0.5 * SW)), Ny - SW), 0)
+    return (x0, y0, z0), front
+
+
+def _gather(a, origin, fill):
+    """a[x0:x0+64, y0:y0+64, z0:z0+64] with `fill` outside the domain."""
+    Nx, Ny, Nz = a.shape
+    x0, y0, z0 = origin
+    out = np.full((SW, SW, SW), fill, dtype=np.float64)
+    xs, ys, zs = slice(x0, min(x0 + SW, Nx)), slice(y0, min(y0 + SW, Ny)), slice(z0, min(z0 + SW, Nz))
+    out[:xs.stop - x0, :ys.stop - y0, :zs.stop - z0] = a[xs, ys, zs]
+    return out
+
+
+def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0):
+    """Collimated beam intensity (relative to its peak) over the window."""
+    Nx, Ny, Nz = mu_a.shape
+    dx, dy, dz = spacing
+    x0, y0, z0 = origin
+    sdir = 1 if math.cos(phi) >= 0 else -1
+    s0 = n_amb * math.sin(theta)
+    w0 = spot / 2.0
+    X = np.arange(x0 - SW, x0 + SW)
+    Y = np.arange(y0, y0 + SW)
+    xa_top = xc + sdir * z0 * dz * math.tan(theta)
+    u = ((X + 0.5) * dx - xa_top) * math.cos(theta)
+    v = (Y + 0.5) * dy - yc
+    Bm = np.exp(-2.0 * (u[:, None] ** 2 + v[None, :] ** 2) / (w0 * w0))   # [128, 64]
+    inx = (X >= 0) & (X < Nx)
+    iny = (Y >= 0) & (Y < Ny)
+    nprev = np.full(Bm.shape, n_amb)
+    out = np.zeros((SW, SW, SW))
+    for k in range(SW):
+        Z = z0 + k
+        ma = np.zeros(Bm.shape)
+        ms = np.zeros(Bm.shape)
+        nk = np.full(Bm.shape, n_amb)
+        if 0 <= Z < Nz:
+            m = inx[:, None] & iny[None, :]
+            xi = np.clip(X, 0, Nx - 1)[:, None]
+            yi = np.clip(Y, 0, Ny - 1)[None, :]
+            ma = np.where(m, mu_a[xi, yi, Z], 0.0)
+            ms = np.where(m, mu_s[xi, yi, Z], 0.0)
+            nk = np.where(m, n[xi, yi, Z], n_amb)
+        # up-beam neighbour (row index t - sdir), own value at the row ends
+        Bn = np.roll(Bm, sdir, axis=0)
+        if sdir > 0:
+            Bn[0] = Bm[0]
+        else:
+            Bn[-1] = Bm[-1]
+        sk = s0 / nk
+        ok = sk < 1.0
+        ck = np.sqrt(np.clip(1.0 - sk * sk, 0.0, None))
+        sh = np.minimum(dz * sk / np.where(ok, ck, 1.0) / dx, 1.0)
+        bi = (1.0 - sh) * Bm + sh * Bn
+        sp = s0 / nprev
+        cp = np.sqrt(np.clip(1.0 - sp * sp, 0.0, None))
+        R = fresnel_u(nprev, nk, cp)
+        Bk = np.where(ok, bi * (1.0 - R) * np.exp(-(ma + ms) * dz / np.where(ok, ck, 1.0)), 0.0)
+        Bm = Bk
+        nprev = nk
+        out[:, :, k] = Bm[SW:, :]
+    return out
+
+
+def features(mu_a, mu_s, g, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0):
+    """[64, 64, 64, 8] float64, the device channel order."""
+    x = np.zeros((SW, SW, SW, 8))
+    ma = _gather(mu_a, origin, 0.0)
+    ms = _gather(mu_s, origin, 0.0)
+    gg = _gather(g, origin, 0.0)
+    nn = _gather(n, origin, n_amb)
+    B = ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi, n_amb)
+    x[..., 0] = np.log1p(ma)
+    x[..., 1] = np.log1p(ms * 0.1)
+    x[..., 2] = gg
+    x[..., 3] = nn - 1.0
+    x[..., 4] = np.log(np.maximum(B, 1e-6)) / LN1E6 + 1.0
+    x[..., 5] = math.sin(theta)
+    x[..., 6] = spot / 0.5 - 1.0
+    x[..., 7] = np.arange(SW)[None, None, :] / 63.0
+    return x
+
+
+def targets(L, mu_a, spacing, origin, n_photons, spot):
+    """(t, w) of the window from the path-length tally L (MC units)."""
+    dx, dy, dz = spacing
+    w0 = spot / 2.0
+    scale = math.pi * w0 * w0 / 2.0 / (n_photons * dx * dy * dz)
+    phi = _gather(L, origin, 0.0) * scale
+    q = _gather(mu_a, origin, 0.0) * phi
+    tq = float((q * q).sum())
+    tp = float(phi.sum())
+    uni = 1.0 / SW ** 3
+    w = (q * q * 0.5 / tq if tq > 0 else 0.5 * uni) + (phi * 0.5 / tp if tp > 0 else 0.5 * uni)
+    return np.log(phi + PHI_EPS), w
+
+
+def _ramp(lo, hi):
+    i = np.arange(SW) + 0.5
+    r = np.ones(SW)
+    if lo:
+        r = np.minimum(r, i / RAMP)
+    if hi:
+        r = np.minimum(r, (SW - i) / RAMP)
+    return r
+
+
+def apply(y, mu_a, Qs, origin, power, spot):
+    """Composed absorbed-power field (float64, domain shape)."""
+    Nx, Ny, Nz = mu_a.shape
+    x0, y0, z0 = origin
+    w0 = spot / 2.0
+    E0 = power / (math.pi * w0 * w0 / 2.0)
+    out = np.array(Qs, dtype=np.float64, copy=True)
+    n = (Nx, Ny, Nz)
+    o = (x0, y0, z0)
+    rs = [_ramp(o[d] > 0, o[d] + SW < n[d]) for d in range(3)]
+    r = rs[0][:, None, None] * rs[1][None, :, None] * rs[2][None, None, :]
+    ex, ey, ez = min(SW, Nx - x0), min(SW, Ny - y0), min(SW, Nz - z0)
+    ma = mu_a[x0:x0 + ex, y0:y0 + ey, z0:z0 + ez]
+    qs = E0 * ma * np.maximum(np.exp(y[:ex, :ey, :ez]) - PHI_EPS, 0.0)
+    rr = r[:ex, :ey, :ez]
+    out[x0:x0 + ex, y0:y0 + ey, z0:z0 + ez] = rr * qs + (1.0 - rr) * out[x0:x0 + ex, y0:y0 + ey, z0:z0 + ez]
+    return out
+
+
+def rel_error(Qm, Qs, ablated):
+    """(per-voxel, 4^3-block) relative L2 of Qs against Qm over non-ablated voxels."""
+    keep = ~ablated
+    a = np.where(keep, Qm, 0.0)
+    b = np.where(keep, Qs, 0.0)
+    e0 = math.sqrt(((a - b) ** 2).sum() / (a ** 2).sum())
+    Nx, Ny, Nz = a.shape
+    px, py, pz = (-Nx) % 4, (-Ny) % 4, (-Nz) % 4
+    ap = np.pad(a, ((0, px), (0, py), (0, pz)))
+    bp = np.pad(b, ((0, px), (0, py), (0, pz)))
+    sa = ap.reshape(ap.shape[0] // 4, 4, ap.shape[1] // 4, 4, ap.shape[2] // 4, 4).sum(axis=(1, 3, 5))
+    sb = bp.reshape(bp.shape[0] // 4, 4, bp.shape[1] // 4, 4, bp.shape[2] // 4, 4).sum(axis=(1, 3, 5))
+    e1 = math.sqrt(((sa - sb) ** 2).sum() / (sa ** 2).sum())
+    return e0, e1

@@ -1,0 +1,83 @@
+"""Pins of the surrogate-coupling oracle (oracle/surrogate_oracle.py) by closed
+forms: Beer-Lambert with the normal-incidence Fresnel factor, the Snell drift of
+an oblique beam's centroid, target/weight identities, and composition."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import surrogate_oracle as S
+
+SP = (0.015, 0.015, 0.015)
+
+
+def _slab(n_t=1.4, mu_a=0.5, mu_s=3.0, air=0, shape=(64, 64, 64)):
+    ma = np.full(shape, mu_a)
+    ms = np.full(shape, mu_s)
+    g = np.full(shape, 0.9)
+    n = np.full(shape, n_t)
+    ma[:, :, :air] = 0.0
+    ms[:, :, :air] = 0.0
+    n[:, :, :air] = 1.0
+    return ma, ms, g, n
+
+
+def test_beer_lambert_with_fresnel_at_normal_incidence():
+    ma, ms, g, n = _slab()
+    xc = yc = 32 * 0.015
+    B = S.ballistic(ma, ms, n, SP, (0, 0, 0), xc, yc, 0.0, 0.5)
+    R = ((1.0 - 1.4) / 2.4) ** 2
+    I0 = math.exp(-2.0 * (2 * (0.5 * 0.015) ** 2) / 0.25 ** 2)   # voxel (32, 32) centre is half a voxel off axis
+    k = np.arange(64)
+    want = I0 * (1 - R) * np.exp(-(0.5 + 3.0) * 0.015 * (k + 1))
+    np.testing.assert_allclose(B[32, 32, :], want, rtol=1e-12)
+
+
+def test_oblique_centroid_drifts_by_snell():
+    ma, ms, g, n = _slab(n_t=1.5, mu_a=0.0, mu_s=0.0, air=10)
+    th = math.radians(30.0)
+    B = S.ballistic(ma, ms, n, SP, (0, 0, 0), 20 * 0.015, 32 * 0.015, th, 0.06)
+    x = np.arange(64)
+    cen = [(B[:, 32, k] * x).sum() / B[:, 32, k].sum() for k in range(40)]
+    d = np.diff(cen)
+    assert d[3] == pytest.approx(math.tan(th), rel=1e-9)                      # in air
+    tt = math.asin(math.sin(th) / 1.5)
+    assert d[25] == pytest.approx(math.tan(tt), rel=1e-9)                     # in the n = 1.5 medium
+
+
+def test_targets_identities():
+    rng = np.random.default_rng(0)
+    L = rng.uniform(0.0, 5.0, (70, 66, 64))
+    ma = rng.uniform(0.0, 2.0, (70, 66, 64))
+    t, w = S.targets(L, ma, SP, (3, 1, 0), 1e6, 0.5)
+    assert w.sum() == pytest.approx(1.0, rel=1e-12)
+    scale = math.pi * 0.25 ** 2 / 2 / (1e6 * 0.015 ** 3)
+    np.testing.assert_allclose(np.exp(t) - 1e-4, L[3:67, 1:65, :] * scale, rtol=1e-10, atol=1e-12)
+
+
+def test_apply_is_consistent_and_blends_to_the_stale_field():
+    rng = np.random.default_rng(1)
+    shape = (100, 90, 80)
+    ma = rng.uniform(0.1, 2.0, shape)
+    Qs = rng.uniform(0.0, 10.0, shape)
+    o = (20, 13, 5)
+    E0 = 40.0 / (math.pi * 0.25 ** 2 / 2)
+    # a prediction that reproduces the stale field leaves it unchanged
+    y = np.log(Qs[20:84, 13:77, 5:69] / (E0 * ma[20:84, 13:77, 5:69]) + 1e-4)
+    np.testing.assert_allclose(S.apply(y, ma, Qs, o, 40.0, 0.5), Qs, rtol=1e-10)
+    # an arbitrary prediction: exact inside, untouched outside
+    y2 = rng.standard_normal((64, 64, 64))
+    out = S.apply(y2, ma, Qs, o, 40.0, 0.5)
+    c = (20 + 32, 13 + 32, 5 + 32)
+    assert out[c] == pytest.approx(E0 * ma[c] * max(math.exp(y2[32, 32, 32]) - 1e-4, 0.0), rel=1e-12)
+    assert out[0, 0, 0] == Qs[0, 0, 0] and out[99, 89, 79] == Qs[99, 89, 79]
+
+
+def test_window_finds_the_front():
+    ma, ms, g, n = _slab(air=21, shape=(128, 96, 100))
+    abl = np.zeros(ma.shape, bool)
+    abl[:, :, 21:40] = True
+    o, front = S.window(ma, ms, abl, SP, 64 * 0.015, 48 * 0.015, math.radians(22.0), margin=12)
+    assert front == 40 and o[2] == 28
+    assert o[1] == 16
