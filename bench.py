@@ -1,13 +1,19 @@
-"""Benchmark: simulated milliseconds per wall second of the coupled epoch loop.
+"""Benchmark: simulated milliseconds per wall second of the coupled epoch loop,
+with the surrogate in the loop against the plain MC loop.
 
 Workload: the coupled layered-stack run at 256x256x192 voxels of 15 um (seeded
 7-layer stack under 21 voxels of ambient air, 40 W at 22 deg, 0.5 mm spot,
 1 ms epochs of 150k photons + 20 sub-steps of 50 us, boil 100 degC, ablate
 350 degC, Arrhenius char, diffusion plume on a 128^3 grid), synthetic inputs,
 state resident in HBM before the timed region.  One step = one epoch = 1 ms
-simulated.  Prints one JSON line.
+simulated.  Two runs from the same initial state and seeds: the plain MC loop,
+and the surrogate loop (network on 9 of 10 epochs, MC judging every 10th and
+taking over while the judged error exceeds 5%).  ``value`` is the surrogate
+loop's rate; the plain loop's rate, the judged errors, and the crater depth,
+ablated volume and peak temperature of both runs are reported beside it.
+Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--plain-only]
 """
 
 import argparse
@@ -24,6 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per voxel of one fused sub-step with char on, Arrhenius off:
 # read T, k, rho_cp, Q, write T (20 B) + H r/w (8) + chi r/w (8) + dry (1) + ablated (1)
 SUBSTEP_BYTES_PER_VOXEL = 38
+WEIGHTS = os.path.join(ROOT, "bug-defense_amd", "surrogate", "weights", "fno_layered_15um.npz")
 
 
 def log(msg):
@@ -44,6 +51,79 @@ def cpu_baseline(kw, epochs, threads):
     return epochs * eng.dt_epoch * 1e3 / dt, dt
 
 
+def final_state(eng, n_air_top):
+    """Crater depth below the initial surface [mm], ablated volume [mm^3], peak T [degC]."""
+    import numpy as np
+    abl = np.array(eng.ablated, copy=True)
+    abl[:, :, :n_air_top] = False
+    v = float(abl.sum()) * eng.dx * eng.dy * eng.dz
+    zs = np.nonzero(abl.any(axis=(0, 1)))[0]
+    depth = float((zs.max() + 1 - n_air_top) * eng.dz) if zs.size else 0.0
+    return {"crater_depth_mm": round(depth, 4), "ablated_volume_mm3": round(v, 5),
+            "peak_T_C": round(float(eng.T_peak.max()), 2)}
+
+
+def run_plain(kw, steps, warmup, graph):
+    import numpy as np
+    from simulation.coupled_ablation import CoupledAblationEngine
+    eng = CoupledAblationEngine(graph=graph, **kw)
+    for _ in range(warmup):
+        eng.step_epoch()
+    log(f"bench: plain loop, {warmup} warmup epochs done")
+    eng._ctx.sync()
+    rec = {k: [] for k in ("sub", "mc", "opt", "th", "ph")}
+    events, nnew = 0.0, 0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        d = eng.step_epoch()
+        ld = eng.last_diag
+        rec["sub"].append(ld.t_substeps_ms)
+        rec["mc"].append(ld.t_mc_kernel_ms)
+        rec["opt"].append(ld.t_optical_ms)
+        rec["th"].append(ld.t_thermal_ms)
+        rec["ph"].append(ld.t_phase_ms)
+        events += ld.mc_events
+        nnew += d['n_new']
+        if (i + 1) % 25 == 0:
+            log(f"bench: plain epoch {i + 1}/{steps}")
+    eng._ctx.sync()
+    wall = time.perf_counter() - t0
+    out = {"wall": wall, "events": events, "nnew": nnew, "nvox": eng.Nx * eng.Ny * eng.Nz,
+           "n_sub": eng.n_substeps, "dt_epoch": eng.dt_epoch, "n_photons": eng.n_photons,
+           **{k: np.asarray(v) for k, v in rec.items()}, "state": final_state(eng, 21)}
+    eng.close()
+    return out
+
+
+def run_surrogate(kw, steps, warmup, weights):
+    import numpy as np
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from surrogate.coupled import SurrogateCoupledEngine
+    eng = CoupledAblationEngine(**kw)
+    sur = SurrogateCoupledEngine(eng, weights, judge_every=10, tol=0.05)
+    for _ in range(warmup):
+        sur.step_epoch()
+    log(f"bench: surrogate loop, {warmup} warmup epochs done")
+    eng._ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        sur.step_epoch()
+        if (i + 1) % 25 == 0:
+            log(f"bench: surrogate epoch {i + 1}/{steps}")
+    eng._ctx.sync()
+    wall = time.perf_counter() - t0
+    hist = sur.history
+    judged = [h["judge_error"] for h in hist if h["judge_error"] is not None]
+    timed = hist[warmup:]
+    out = {"wall": wall, "surrogate_epochs": sum(1 for h in timed if h["optics_source"] == "surrogate"),
+           "mc_epochs": sum(1 for h in timed if h["optics_source"] == "mc"),
+           "judged_block4": [round(e[1], 4) for e in judged], "judged_voxel": [round(e[0], 4) for e in judged],
+           "state": final_state(eng, 21)}
+    sur.close()
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,7 +131,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-epochs", type=int, default=3)
-    ap.add_argument("--graph", action="store_true", help="replay each epoch as a captured HIP graph")
+    ap.add_argument("--graph", action="store_true", help="replay each plain epoch as a captured HIP graph")
+    ap.add_argument("--plain-only", action="store_true", help="time the plain MC loop only")
+    ap.add_argument("--weights", default=WEIGHTS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
@@ -60,44 +142,26 @@ def main():
 
     import numpy as np
     from simulation import _bdlib as B
-    from simulation.coupled_ablation import CoupledAblationEngine
     from simulation.workloads import layered_stack
 
     if B.device_count() < 1:
         raise RuntimeError("bench: no HIP device visible")
     kw = layered_stack()
-    t0 = time.perf_counter()
-    eng = CoupledAblationEngine(graph=args.graph, **kw)
-    log(f"bench: engine built in {time.perf_counter() - t0:.1f}s ({eng.Nx}x{eng.Ny}x{eng.Nz})")
-    for i in range(args.warmup):
-        eng.step_epoch()
-    log(f"bench: {args.warmup} warmup epochs done")
-    eng._ctx.sync()
-    sub_ms, mc_ms, opt_ms, th_ms, ph_ms, events, nnew = [], [], [], [], [], 0.0, 0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        d = eng.step_epoch()
-        ld = eng.last_diag
-        sub_ms.append(ld.t_substeps_ms)
-        mc_ms.append(ld.t_mc_kernel_ms)
-        opt_ms.append(ld.t_optical_ms)
-        th_ms.append(ld.t_thermal_ms)
-        ph_ms.append(ld.t_phase_ms)
-        events += ld.mc_events
-        nnew += d['n_new']
-        if (i + 1) % 25 == 0:
-            log(f"bench: epoch {i + 1}/{args.steps}")
-    eng._ctx.sync()
-    wall = time.perf_counter() - t0
-    sim_ms_per_s = args.steps * eng.dt_epoch * 1e3 / wall
-    nvox = eng.Nx * eng.Ny * eng.Nz
-    n_sub = eng.n_substeps
-    sub_launch_ms = float(np.mean(sub_ms)) / n_sub
-    achieved = SUBSTEP_BYTES_PER_VOXEL * nvox / (sub_launch_ms * 1e-3) / 1e9
-    mc_avg = float(np.mean(mc_ms))
+    p = run_plain(kw, args.steps, args.warmup, args.graph)
+    plain_rate = args.steps * p["dt_epoch"] * 1e3 / p["wall"]
+    sub_launch_ms = float(np.mean(p["sub"])) / p["n_sub"]
+    achieved = SUBSTEP_BYTES_PER_VOXEL * p["nvox"] / (sub_launch_ms * 1e-3) / 1e9
+    mc_avg = float(np.mean(p["mc"]))
+    s = None
+    if not args.plain_only:
+        if not os.path.exists(args.weights):
+            raise RuntimeError(f"bench: surrogate weights not found at {args.weights}")
+        s = run_surrogate(kw, args.steps, args.warmup, args.weights)
+    rate = args.steps * p["dt_epoch"] * 1e3 / (s["wall"] if s else p["wall"])
+    wall = s["wall"] if s else p["wall"]
     result = {
-        "metric": "simulated ms/s at 256x256x192 (plain MC path); photons/s",
-        "value": round(sim_ms_per_s, 3),
+        "metric": "simulated ms/s at 256x256x192: surrogate+MC vs plain MC at <=5% L2; photons/s",
+        "value": round(rate, 3),
         "unit": "simulated ms/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -106,20 +170,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (transport, stencils); f64 ledgers",
-        "data": "synthetic (seeded layered stack, no datasets)",
+        "dtype": "f32 (transport, stencils); bf16 MFMA / f32 accumulate (surrogate); f64 ledgers",
+        "data": "synthetic (seeded layered stack, no datasets); surrogate weights trained on device-MC samples",
         "config": {
-            "workload": "coupled layered stack, plain MC path: 256x256x192 voxels of 15 um, 7 layers + 21 voxels "
-                        "ambient air, 40 W at 22 deg, 0.5 mm spot, 1 ms epochs of 150k photons + 20 x 50 us "
-                        "sub-steps, boil 100 C, ablate 350 C, Arrhenius char, diffusion plume on 128^3",
-            "voxels": nvox, "photons_per_epoch": eng.n_photons, "substeps_per_epoch": n_sub,
+            "workload": "coupled layered stack: 256x256x192 voxels of 15 um, 7 layers + 21 voxels ambient air, "
+                        "40 W at 22 deg, 0.5 mm spot, 1 ms epochs of 150k photons + 20 x 50 us sub-steps, boil "
+                        "100 C, ablate 350 C, Arrhenius char, diffusion plume on 128^3" +
+                        ("" if args.plain_only else "; surrogate on 64^3 beam window, MC judge every 10th epoch, "
+                                                    "fallback above 5% L2"),
+            "voxels": p["nvox"], "photons_per_epoch": p["n_photons"], "substeps_per_epoch": p["n_sub"],
+            "mode": "plain MC" if args.plain_only else "surrogate+MC",
         },
-        "photons_per_s": round(eng.n_photons / (mc_avg * 1e-3), 1),
-        "mc_events_per_s": round(events / (sum(mc_ms) * 1e-3), 1),
-        "breakdown_ms": {"optical": round(float(np.mean(opt_ms)), 4), "thermal": round(float(np.mean(th_ms)), 4),
-                         "substeps": round(float(np.mean(sub_ms)), 4), "mc_kernel": round(mc_avg, 4),
-                         "phase_plume": round(float(np.mean(ph_ms)), 4)},
-        "removed_voxels": int(nnew),
+        "plain_mc": {"value": round(plain_rate, 3), "ms_per_step": round(p["wall"] * 1e3 / args.steps, 4),
+                     **p["state"]},
+        "photons_per_s": round(p["n_photons"] / (mc_avg * 1e-3), 1),
+        "mc_events_per_s": round(p["events"] / (float(np.sum(p["mc"])) * 1e-3), 1),
+        "breakdown_ms": {"optical": round(float(np.mean(p["opt"])), 4), "thermal": round(float(np.mean(p["th"])), 4),
+                         "substeps": round(float(np.mean(p["sub"])), 4), "mc_kernel": round(mc_avg, 4),
+                         "phase_plume": round(float(np.mean(p["ph"])), 4)},
         "roofline": {
             "kernel": "k_substep (fused bioheat sub-step)",
             "bound": "hbm",
@@ -132,7 +200,12 @@ def main():
             "avg_launch_ms": round(sub_launch_ms, 5),
         },
     }
-    eng.close()
+    if s:
+        jb = s["judged_block4"]
+        result["surrogate"] = {"value": round(rate, 3), "speedup_vs_plain": round(rate / plain_rate, 3),
+                               "surrogate_epochs": s["surrogate_epochs"], "mc_epochs": s["mc_epochs"],
+                               "judged_rel_l2_block4": jb, "judged_rel_l2_voxel": s["judged_voxel"],
+                               "max_judged_rel_l2_block4": max(jb) if jb else None, **s["state"]}
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"bench: cpu baseline ({args.cpu_epochs} epochs, {threads} threads)")

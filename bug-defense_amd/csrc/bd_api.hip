@@ -2,6 +2,8 @@
 // field transfer and the per-call orchestration of the kernels.
 #include "bd_internal.hpp"
 
+#include <cstdlib>
+
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -140,6 +142,7 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     BD_HIP(hipEventCreate(&c->ev4));
     BD_HIP(hipEventCreate(&c->ev5));
     for (int i = 0; i < 6; ++i) BD_HIP(hipEventCreate(&c->gev[i]));
+    if (const char *e = std::getenv("BUGDEF_MC_BLOCKS_PER_CU")) c->mc_blocks_per_cu = std::max(0, std::atoi(e));
     BD_HIP(hipHostMalloc(&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
     BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + 1024)));
     BD_TRY(ensure(c, (void **)&c->counter, sizeof(unsigned long long) * 16));

@@ -209,10 +209,10 @@ static bool same_beam(const bd_beam &a, const bd_beam &b)
 
 }  // namespace bd
 
-extern "C" int bd_epoch_graph(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_before, int64_t epoch,
-                              uint64_t seed_base, bd_epoch_diag *out)
+extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_before,
+                                     int64_t epoch, uint64_t seed_base)
 {
-    BD_CHECK(h && beam && out, "bd_epoch_graph: null argument");
+    BD_CHECK(h && beam, "bd_epoch_graph: null argument");
     BD_CHECK(n_substeps >= 1, "bd_epoch_graph: n_substeps must be >= 1");
     Ctx *c = &h->c;
     if (!c->has_phys) {
@@ -274,7 +274,11 @@ extern "C" int bd_epoch_graph(bd_ctx *h, const bd_beam *beam, int n_substeps, do
         c->graph_beam = *beam;
         c->graph_nsub = n_substeps;
     }
-    // per-epoch scalars -> device (outside the graph), replay, one readback
+    // per-epoch scalars -> device (outside the graph), replay; the readback is bd_epoch_graph_finish
+    if (c->graph_pending) {
+        set_error("bd_epoch_graph_launch: the previous epoch graph was not finished");
+        return BD_ESTATE;
+    }
     const double dt_epoch = n_substeps * c->phys.dt_sub;
     EpochParams *hp = (EpochParams *)((char *)c->host_pinned + 56 * sizeof(double));
     hp->seed_base = seed_base;
@@ -285,6 +289,24 @@ extern "C" int bd_epoch_graph(bd_ctx *h, const bd_beam *beam, int n_substeps, do
     BD_HIP(hipMemcpyAsync(c->ep, hp, sizeof(EpochParams), hipMemcpyHostToDevice, c->stream));
     BD_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     BD_HIP(hipEventRecord(c->ev3, c->stream));
+    c->graph_pending = true;
+    c->pending_beam = *beam;
+    c->pending_dt_epoch = dt_epoch;
+    return BD_OK;
+}
+
+// wait for a launched epoch graph and parse its diagnostics
+extern "C" int bd_epoch_graph_finish(bd_ctx *h, bd_epoch_diag *out)
+{
+    BD_CHECK(h && out, "bd_epoch_graph_finish: null argument");
+    Ctx *c = &h->c;
+    if (!c->graph_pending) {
+        set_error("bd_epoch_graph_finish: no epoch graph was launched");
+        return BD_ESTATE;
+    }
+    c->graph_pending = false;
+    const bd_beam *beam = &c->pending_beam;
+    const double dt_epoch = c->pending_dt_epoch;
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;
     const unsigned long long *cnt = (const unsigned long long *)(r + 48);
@@ -335,4 +357,12 @@ extern "C" int bd_epoch_graph(bd_ctx *h, const bd_beam *beam, int n_substeps, do
         out->t_mc_kernel_ms = ms;
     }
     return BD_OK;
+}
+
+extern "C" int bd_epoch_graph(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_before, int64_t epoch,
+                              uint64_t seed_base, bd_epoch_diag *out)
+{
+    BD_CHECK(h && beam && out, "bd_epoch_graph: null argument");
+    BD_TRY(bd_epoch_graph_launch(h, beam, n_substeps, t_before, epoch, seed_base));
+    return bd_epoch_graph_finish(h, out);
 }

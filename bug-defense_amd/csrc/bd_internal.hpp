@@ -92,6 +92,9 @@ struct Ctx {
     size_t staging_bytes = 0;
     void *host_pinned = nullptr;         // small pinned host block (64 doubles)
 
+    // persistent MC grid: blocks of 256 per CU (0: one lane per photon up to 8 blocks per CU)
+    int mc_blocks_per_cu = 0;
+
     // tuned launch shape of the sub-step kernel
     int tile_by = 4, tile_xc = 16;
     bool tuned = false;
@@ -104,6 +107,9 @@ struct Ctx {
     bd_beam graph_beam{};
     int graph_nsub = -1;
     const float *graph_T = nullptr, *graph_P = nullptr;  // ping-pong heads the graph was captured with
+    bool graph_pending = false;          // launched by bd_epoch_graph_launch, not yet finished
+    bd_beam pending_beam{};
+    double pending_dt_epoch = 0.0;
 };
 
 void set_error(const std::string &msg);

@@ -586,7 +586,8 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
     int ncu = 256;
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     const int64_t want = (b->n_photons + 255) / 256;
-    const int64_t cap = (int64_t)ncu * 8;
+    int64_t cap = (int64_t)ncu * 8;
+    if (c->mc_blocks_per_cu > 0) cap = (int64_t)ncu * c->mc_blocks_per_cu;
     const int nblk = (int)(want < cap ? (want < 1 ? 1 : want) : cap);
     if (timed == 1) BD_HIP(hipEventRecord(c->ev0, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[0], c->stream, hipEventRecordExternal));

@@ -14,7 +14,10 @@
 //     (planar Snell invariant n sin(theta), Fresnel transmission at index
 //     steps, Beer-Lambert attenuation mu_t dz / cos(theta), lateral drift
 //     dz tan(theta) by linear interpolation)
-//   5 sin(theta_in)   6 spot / 0.5 mm - 1   7 k / 63 (depth in the window)
+//   5 sin(theta_in)   6 spot / 0.5 mm - 1
+//   7 optical depth along the beam from the window top, ln(1 + tau) / ln(101),
+//     carried with the same drift as channel 4 (it keeps growing where the
+//     beam intensity has fallen below the channel-4 floor)
 // Optics are the packed transport optics the MC itself reads (plume included).
 //
 // Target: t = ln(Phi / E0 + 1e-4) with Phi = L / (N V) * P the fluence rate and
@@ -110,10 +113,11 @@ __global__ __launch_bounds__(128) void k_sur_features(FeatArgs a)
     const SurGeom &g = a.g;
     const int j = blockIdx.x, t = threadIdx.x;
     const int X = g.x0 - SW + t, Y = g.y0 + j;
-    __shared__ float sB[128];
+    __shared__ float sB[128], sT[128];
     const float xm = (X + 0.5f) * g.dx, ym = (Y + 0.5f) * g.dy;
     const float u = (xm - a.xa_top) * a.cos0, v = ym - a.yc;
     float B = expf(-2.f * (u * u + v * v) / (a.w0 * a.w0));
+    float tau = 0.f;
     float nprev = a.n_amb;
     const float s0 = a.n_amb * a.sin0;
     for (int k = 0; k < SW; ++k) {
@@ -121,12 +125,15 @@ __global__ __launch_bounds__(128) void k_sur_features(FeatArgs a)
         float4 o = make_float4(0.f, 0.f, 0.f, a.n_amb);
         if (in_dom(g, X, Y, Z)) o = g.opt[((int64_t)X * g.Ny + Y) * g.Nz + Z];
         sB[t] = B;
+        sT[t] = tau;
         __syncthreads();
         const int tn = t - a.sdir;
-        const float Bn = (tn >= 0 && tn < 128) ? sB[tn] : B;
+        const bool nb_in = tn >= 0 && tn < 128;
+        const float Bn = nb_in ? sB[tn] : B;
+        const float Tn = nb_in ? sT[tn] : tau;
         __syncthreads();
         const float sk = s0 / o.w;
-        float Bk = 0.f;
+        float Bk = 0.f, tk = tau + (o.x + o.y) * g.dz;
         if (sk < 1.f) {
             const float ck = sqrtf(1.f - sk * sk);
             const float sh = fminf(g.dz * sk / (ck * g.dx), 1.f);
@@ -135,8 +142,10 @@ __global__ __launch_bounds__(128) void k_sur_features(FeatArgs a)
             const float cp = sqrtf(fmaxf(0.f, 1.f - sp * sp));
             const float R = fresnel_u(nprev, o.w, cp);
             Bk = bi * (1.f - R) * expf(-(o.x + o.y) * g.dz / ck);
+            tk = (1.f - sh) * tau + sh * Tn + (o.x + o.y) * g.dz / ck;
         }
         B = Bk;
+        tau = tk;
         nprev = o.w;
         if (t >= SW) {
             float *xp = a.x + (((int64_t)(t - SW) * SW + j) * SW + k) * 8;
@@ -147,7 +156,7 @@ __global__ __launch_bounds__(128) void k_sur_features(FeatArgs a)
             xp[4] = logf(fmaxf(B, 1e-6f)) * (1.f / 13.815510557964274f) + 1.f;
             xp[5] = a.f5;
             xp[6] = a.f6;
-            xp[7] = k * (1.f / 63.f);
+            xp[7] = log1pf(tau) * (1.f / 4.61512051684126f);
         }
     }
 }

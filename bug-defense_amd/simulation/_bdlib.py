@@ -131,6 +131,9 @@ _SIGS = {
                   ctypes.POINTER(EpochDiag)], ctypes.c_int),
     "bd_epoch_graph": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double, ctypes.c_int64,
                         ctypes.c_uint64, ctypes.POINTER(EpochDiag)], ctypes.c_int),
+    "bd_epoch_graph_launch": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double,
+                               ctypes.c_int64, ctypes.c_uint64], ctypes.c_int),
+    "bd_epoch_graph_finish": ([ctypes.c_void_p, ctypes.POINTER(EpochDiag)], ctypes.c_int),
     "bd_reduce": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                   ctypes.c_int),
     "bd_phase_map": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
@@ -440,6 +443,15 @@ class Context:
         d = EpochDiag()
         check(lib().bd_epoch_graph(self.handle, ctypes.byref(beam), int(n_substeps), float(t_before), int(epoch),
                                    int(seed_base) & 0xFFFFFFFFFFFFFFFF, ctypes.byref(d)), "bd_epoch_graph")
+        return d
+
+    def epoch_graph_launch(self, beam: Beam, n_substeps, t_before, epoch, seed_base):
+        check(lib().bd_epoch_graph_launch(self.handle, ctypes.byref(beam), int(n_substeps), float(t_before),
+                                          int(epoch), int(seed_base) & 0xFFFFFFFFFFFFFFFF), "bd_epoch_graph_launch")
+
+    def epoch_graph_finish(self) -> EpochDiag:
+        d = EpochDiag()
+        check(lib().bd_epoch_graph_finish(self.handle, ctypes.byref(d)), "bd_epoch_graph_finish")
         return d
 
     def epoch_q(self, beam: Beam, n_substeps, t_after) -> EpochDiag:

@@ -467,14 +467,33 @@ class CoupledAblationEngine:
         beam = self._epoch_beam()
         t_after = self.t + self.dt_epoch
         t0 = time.perf_counter()
-        graphable = (self.graph and self.n_substeps % 2 == 0 and abs(self.n_substeps * self.dt_sub - self.dt_epoch)
-                     <= 1e-12 * self.dt_epoch and (self.plume is None or (self.plume.model == 'diffusion'
-                                                                          and not self.plume.blast_expansion)))
-        if graphable:
+        if self.graph and self.graphable():
             # same MC key (mix_seed(seed, epoch)) and removal time (t + dt_epoch), formed on the device
             d = self._ctx.epoch_graph(beam, self.n_substeps, self.t, self.epoch, self.seed)
         else:
             d = self._ctx.epoch(beam, self.n_substeps, t_after)
+        return self._finish_epoch(d, t_after, time.perf_counter() - t0)
+
+    def graphable(self) -> bool:
+        return (self.n_substeps % 2 == 0 and abs(self.n_substeps * self.dt_sub - self.dt_epoch)
+                <= 1e-12 * self.dt_epoch and (self.plume is None or (self.plume.model == 'diffusion'
+                                                                     and not self.plume.blast_expansion)))
+
+    def step_epoch_launch(self):
+        """Queue one graph-captured epoch without waiting (several engines can
+        then have epochs in flight on their own streams); finish it with
+        step_epoch_finish before any other call on this engine."""
+        beam = self._epoch_beam()
+        if not self.graphable():
+            raise ValueError("step_epoch_launch needs a graph-capturable configuration (even sub-step count, "
+                             "diffusion plume without blast expansion)")
+        self._pending = (self.t + self.dt_epoch, time.perf_counter())
+        self._ctx.epoch_graph_launch(beam, self.n_substeps, self.t, self.epoch, self.seed)
+
+    def step_epoch_finish(self) -> dict:
+        t_after, t0 = self._pending
+        d = self._ctx.epoch_graph_finish()
+        self._pending = None
         return self._finish_epoch(d, t_after, time.perf_counter() - t0)
 
     def step_epoch_with_q(self) -> dict:

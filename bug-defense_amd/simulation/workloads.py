@@ -20,19 +20,16 @@ STACK_UM = (20.0, 300.0, 80.0, 150.0, 200.0, 400.0, 60.0)
 AIR_LAYER = 1
 
 
-def layered_stack(shape=(256, 256, 192), voxel_mm=0.015, n_air_top=21, seed=0, power=40.0, theta_deg=22.0,
-                  spot_dia=0.5, n_photons=150_000, dt_epoch=1e-3, dt_sub=5e-5, plume_grid=(128, 128, 128)):
-    """Engine keywords for the coupled layered-stack configuration.
+def stack_engine_kwargs(shape, voxel_mm, n_air_top, layers, power, theta_deg, spot_dia, n_photons, dt_epoch,
+                        dt_sub, plume_grid, seed, wavelength_nm=532.0):
+    """Engine keywords for a planar stack under ambient air.
 
-    Per layer, from ``np.random.default_rng(seed)``: mu_a U[0.01, 20] /mm,
-    mu_s U[1, 40] /mm, g U[0.7, 0.95], n U[1.0, 1.55], k U[0.1, 0.6] W/(m K),
-    water fraction w U[0, 0.8]; rho_cp = w 4.184e-3 + (1 - w) 2.1e-3 J/(mm^3 K);
-    the outermost layer's w is scaled by 0.1 (a cuticle); layers with w < 0.2
-    are dry (h_lat 0, char, removal at 350 degC), the others boil with
-    h_lat = 2.26 w J/mm^3.  The deepest layer continues to the bottom.
+    layers: per entry of STACK_UM, None for the air gap or a tuple
+    (mu_a [1/mm], mu_s [1/mm], g, n, k [W/(mm K)], rho_cp [J/(mm^3 K)], water fraction);
+    layers with water fraction < 0.2 are dry (h_lat 0, char, removal at 350 degC),
+    the others boil with h_lat = 2.26 w J/mm^3.  The deepest layer continues to the bottom.
     """
     Nx, Ny, Nz = shape
-    rng = np.random.default_rng(seed)
     mu_a = np.zeros(shape)
     mu_s = np.zeros(shape)
     g = np.zeros(shape)
@@ -48,17 +45,14 @@ def layered_stack(shape=(256, 256, 192), voxel_mm=0.015, n_air_top=21, seed=0, p
     for i, um in enumerate(STACK_UM):
         nvox = max(1, int(round(um / (voxel_mm * 1000.0))))
         z1 = Nz if i == len(STACK_UM) - 1 else min(Nz, z + nvox)
-        draw = (rng.uniform(0.01, 20.0), rng.uniform(1.0, 40.0), rng.uniform(0.7, 0.95), rng.uniform(1.0, 1.55),
-                rng.uniform(0.1, 0.6) * 1e-3, rng.uniform(0.0, 0.8))
-        if i == AIR_LAYER:
+        lay = layers[i]
+        if lay is None:
             gap = (z, z1)
         else:
-            ma, ms, gg, nn, kk, w = draw
-            if i == 0:
-                w *= 0.1  # the outermost layer is cuticle: dry, it chars and ablates at 350 degC
+            ma, ms, gg, nn, kk, rcc, w = lay
             mu_a[:, :, z:z1], mu_s[:, :, z:z1], g[:, :, z:z1], n[:, :, z:z1] = ma, ms, gg, nn
             k[:, :, z:z1] = kk
-            rc[:, :, z:z1] = w * 4.184e-3 + (1.0 - w) * 2.1e-3
+            rc[:, :, z:z1] = rcc
             is_dry = w < 0.2
             hl[:, :, z:z1] = 0.0 if is_dry else H_LAT_WATER * w
             dry[:, :, z:z1] = is_dry
@@ -75,5 +69,31 @@ def layered_stack(shape=(256, 256, 192), voxel_mm=0.015, n_air_top=21, seed=0, p
         plume_closure={'model': 'diffusion', 'D_xy': 1.0, 'D_z': 2.0, 'v_drift': 0.0, 'tau_linger_s': float('inf'),
                        'blast_expansion': False, 'blast_sigma_xy_mm': 0.080, 'blast_sigma_z_mm': 0.090,
                        'blast_z_bias_mm': 0.0},
-        initial_dry_mask=dry, initial_ablated_mask=abl, air_gap_range=gap, wavelength_nm=532.0,
+        initial_dry_mask=dry, initial_ablated_mask=abl, air_gap_range=gap, wavelength_nm=wavelength_nm,
         plume_grid=plume_grid, seed=seed + 1)
+
+
+def layered_stack(shape=(256, 256, 192), voxel_mm=0.015, n_air_top=21, seed=0, power=40.0, theta_deg=22.0,
+                  spot_dia=0.5, n_photons=150_000, dt_epoch=1e-3, dt_sub=5e-5, plume_grid=(128, 128, 128)):
+    """Engine keywords for the coupled layered-stack configuration.
+
+    Per layer, from ``np.random.default_rng(seed)``: mu_a U[0.01, 20] /mm,
+    mu_s U[1, 40] /mm, g U[0.7, 0.95], n U[1.0, 1.55], k U[0.1, 0.6] W/(m K),
+    water fraction w U[0, 0.8]; rho_cp = w 4.184e-3 + (1 - w) 2.1e-3 J/(mm^3 K);
+    the outermost layer's w is scaled by 0.1 (a cuticle); layers with w < 0.2
+    are dry (h_lat 0, char, removal at 350 degC), the others boil with
+    h_lat = 2.26 w J/mm^3.  The deepest layer continues to the bottom.
+    """
+    rng = np.random.default_rng(seed)
+    layers = []
+    for i in range(len(STACK_UM)):
+        ma, ms, gg, nn, kk, w = (rng.uniform(0.01, 20.0), rng.uniform(1.0, 40.0), rng.uniform(0.7, 0.95),
+                                 rng.uniform(1.0, 1.55), rng.uniform(0.1, 0.6) * 1e-3, rng.uniform(0.0, 0.8))
+        if i == AIR_LAYER:
+            layers.append(None)
+            continue
+        if i == 0:
+            w *= 0.1  # the outermost layer is cuticle: dry, it chars and ablates at 350 degC
+        layers.append((ma, ms, gg, nn, kk, w * 4.184e-3 + (1.0 - w) * 2.1e-3, w))
+    return stack_engine_kwargs(shape, voxel_mm, n_air_top, layers, power, theta_deg, spot_dia, n_photons, dt_epoch,
+                               dt_sub, plume_grid, seed)

@@ -87,8 +87,9 @@ class Dataset:
         return (self.X.view(c, (nb, T, T, T, CIN)), self.t.view(c, (nb, T, T, T)), self.w.view(c, (nb, T, T, T)))
 
 
-def generate(n, n_photons=1_000_000, seed=0, progress=None, device=0):
-    """Dataset of n samples from the device MC (seeded, reproducible)."""
+def generate(n, n_photons=1_000_000, seed=0, progress=None, device=0, mc_seed_offset=0):
+    """Dataset of n samples from the device MC (seeded, reproducible); the
+    geometry depends on `seed` only, the MC streams also on mc_seed_offset."""
     rng = np.random.default_rng(seed)
     ds = Dataset(n, device=device)
     ctx = B.Context((T, T, T), (VOXEL_MM,) * 3)
@@ -98,7 +99,7 @@ def generate(n, n_photons=1_000_000, seed=0, progress=None, device=0):
             s = random_sample(rng)
             for name in ('mu_a', 'mu_s', 'g', 'n'):
                 ctx.upload(name.upper(), s[name])
-            beam = sample_beam(s, n_photons, seed * 1_000_003 + i + 1)
+            beam = sample_beam(s, n_photons, seed * 1_000_003 + i + 1 + mc_seed_offset * 7_777_777_777)
             ctx.mc_run(beam)
             ctx.sur_features(beam, origin, ds.X.view(i, (T, T, T, CIN)))
             ctx.sur_targets(beam, origin, ds.t.view(i, (T, T, T)), ds.w.view(i, (T, T, T)))

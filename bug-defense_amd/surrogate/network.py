@@ -18,7 +18,7 @@ C = 32      # width
 L = 4       # Fourier layers
 H = 64      # projection width
 
-DEFAULT_MODES = (8, 8, 4)
+DEFAULT_MODES = (6, 6, 8)
 DEFAULT_NP = 80
 
 

@@ -27,23 +27,42 @@ def q_from(log_phi, feats_mu_a):
     return np.expm1(feats_mu_a) * np.maximum(np.exp(log_phi) - PHI_EPS, 0.0)
 
 
-def evaluate(net, ds, idx0, n, batch):
-    """Relative L2 of Q (prediction vs the sample's MC) on samples [idx0, idx0 + n)."""
-    errs, num, den = [], 0.0, 0.0
+def block4(a):
+    n = a.shape[-1] // 4
+    return a.reshape(a.shape[:-3] + (n, 4, n, 4, n, 4)).sum(axis=(-5, -3, -1))
+
+
+def rel_l2(p, t):
+    s = float((t ** 2).sum())
+    return math.sqrt(float(((p - t) ** 2).sum()) / s) if s > 0 else 0.0
+
+
+def evaluate(net, ds, idx0, n, batch, other=None):
+    """Relative L2 of Q, prediction vs the sample's MC, on samples [idx0, idx0 + n):
+    median per voxel, median over 4^3 blocks, pooled per voxel.  With `other`
+    (the same samples under independent MC streams) also the MC-vs-MC medians."""
+    ev, eb, mv, mb, num, den = [], [], [], [], 0.0, 0.0
     for c in range(idx0 // batch, (idx0 + n) // batch):
         X, t, _ = ds.chunk(c, batch)
-        y = net.forward(X).download()
-        tt = t.download()
-        mu = X.download()[..., 0]
+        y = net.forward(X).download().astype(np.float64)
+        tt = t.download().astype(np.float64)
+        mu = X.download()[..., 0].astype(np.float64)
         qp = q_from(y, mu)
         qt = q_from(tt, mu)
+        qo = q_from(other.chunk(c - idx0 // batch, batch)[1].download().astype(np.float64), mu) if other else None
         for b in range(batch):
-            d = float(((qp[b] - qt[b]) ** 2).sum())
-            s = float((qt[b] ** 2).sum())
-            errs.append(math.sqrt(d / s) if s > 0 else 0.0)
-            num += d
-            den += s
-    return float(np.median(errs)), float(np.mean(errs)), math.sqrt(num / den)
+            ev.append(rel_l2(qp[b], qt[b]))
+            eb.append(rel_l2(block4(qp[b]), block4(qt[b])))
+            num += float(((qp[b] - qt[b]) ** 2).sum())
+            den += float((qt[b] ** 2).sum())
+            if other:
+                mv.append(rel_l2(qo[b], qt[b]))
+                mb.append(rel_l2(block4(qo[b]), block4(qt[b])))
+    out = {"median_voxel": float(np.median(ev)), "median_block4": float(np.median(eb)),
+           "pooled_voxel": math.sqrt(num / den)}
+    if other:
+        out.update(mc_vs_mc_median_voxel=float(np.median(mv)), mc_vs_mc_median_block4=float(np.median(mb)))
+    return out
 
 
 def main(argv=None):
@@ -57,19 +76,26 @@ def main(argv=None):
     ap.add_argument("--wd", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default=None, help="start from saved weights")
+    ap.add_argument("--modes", default="6,6,8", help="kept modes kx,ky,kz")
+    ap.add_argument("--eval-every", type=int, default=10)
+    ap.add_argument("--np-len", type=int, default=80, help="transform length")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     nb = a.batch
     ntr = (a.samples // nb) * nb
     nho = (a.holdout // nb) * nb
     t0 = time.perf_counter()
-    ds = data.generate(ntr + nho, n_photons=a.photons, seed=a.seed, progress=512)
+    ds = data.generate(ntr, n_photons=a.photons, seed=a.seed, progress=1024)
+    ho = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000)
+    ho2 = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000, mc_seed_offset=1)
     t_data = time.perf_counter() - t0
-    print(f"data: {ntr} train + {nho} held-out samples in {t_data:.1f} s", flush=True)
+    print(f"data: {ntr} train + {nho} held-out samples (+ independent MC of the held-out) in {t_data:.1f} s",
+          flush=True)
     if a.init:
         net = FourierSurrogate.load(a.init, max_batch=nb, train=True)
     else:
-        net = FourierSurrogate(max_batch=nb, train=True)
+        modes = tuple(int(v) for v in a.modes.split(","))
+        net = FourierSurrogate(modes=modes, np_len=a.np_len, max_batch=nb, train=True)
         net.set_params(init_params(net.modes, seed=a.seed))
     rng = np.random.default_rng(a.seed + 1)
     nchunk = ntr // nb
@@ -85,16 +111,14 @@ def main(argv=None):
             losses.append(net.train_step(X, t, w, lr, weight_decay=a.wd))
             step += 1
         rec = {"epoch": ep + 1, "loss": float(np.mean(losses)), "s": round(time.perf_counter() - t1, 1)}
-        if (ep + 1) % 5 == 0 or ep + 1 == a.epochs:
-            med, mean, pooled = evaluate(net, ds, ntr, nho, nb)
-            rec.update(heldout_median=med, heldout_mean=mean, heldout_pooled=pooled)
+        if (ep + 1) % a.eval_every == 0 and ep + 1 < a.epochs:
+            rec.update(evaluate(net, ho, 0, nho, nb))
         hist.append(rec)
         print(json.dumps(rec), flush=True)
     t_train = time.perf_counter() - t1
-    med, mean, pooled = evaluate(net, ds, ntr, nho, nb)
     summary = {"train_samples": ntr, "heldout_samples": nho, "photons": a.photons, "epochs": a.epochs,
-               "data_s": round(t_data, 1), "train_s": round(t_train, 1), "heldout_median_rel_l2": med,
-               "heldout_mean_rel_l2": mean, "heldout_pooled_rel_l2": pooled}
+               "modes": list(net.modes), "np_len": net.np_len, "data_s": round(t_data, 1),
+               "train_s": round(t_train, 1), "heldout": evaluate(net, ho, 0, nho, nb, other=ho2)}
     net.save(a.out, history=json.dumps(hist), summary=json.dumps(summary))
     print(json.dumps(summary), flush=True)
     net.close()

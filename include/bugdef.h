@@ -223,6 +223,14 @@ int bd_epoch(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_after, b
 int bd_epoch_graph(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_before, int64_t epoch,
                    uint64_t seed_base, bd_epoch_diag *out);
 
+/* bd_epoch_graph split in two so several contexts (one stream each) can have
+ * epochs in flight at once: _launch refreshes the device parameters and
+ * replays the graph without waiting; _finish waits and fills the diagnostics.
+ * A context takes no other call between the two. */
+int bd_epoch_graph_launch(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_before, int64_t epoch,
+                          uint64_t seed_base);
+int bd_epoch_graph_finish(bd_ctx *ctx, bd_epoch_diag *out);
+
 /* Reductions (FP64): op 0 sum, 1 max, 2 min over a field; optional u8 mask
  * field (-1 for none) restricts the reduction to mask != 0. */
 int bd_reduce(bd_ctx *ctx, int field, int op, int mask_field, double *out);

@@ -87,7 +87,8 @@ int bd_sur_window(bd_ctx *ctx, const bd_beam *beam, int margin, int32_t *origin)
 
 /* Input features of the window (x_dev: float32 [64][64][64][8], channels
  * log1p(mu_a), log1p(mu_s/10), g, n-1, ballistic beam log-intensity,
- * sin(theta), spot/0.5mm - 1, depth/63) from the packed transport optics. */
+ * sin(theta), spot/0.5mm - 1, ln(1 + optical depth along the beam)/ln(101))
+ * from the packed transport optics. */
 int bd_sur_features(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, float *x_dev);
 
 /* Training target t = ln(Phi/E0 + 1e-4) and loss weights (sum one) of the

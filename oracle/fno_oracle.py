@@ -42,9 +42,9 @@ __all__ = ["Spec", "param_shapes", "n_params", "unflatten", "flatten", "tables",
 @dataclass(frozen=True)
 class Spec:
     T: int = 64        # tile edge (voxels)
-    Kx: int = 8        # kept signed modes along x (Kx/2 non-negative, Kx/2 negative)
-    Ky: int = 8
-    Kz: int = 4        # kept non-negative modes along z
+    Kx: int = 6        # kept signed modes along x (Kx/2 non-negative, Kx/2 negative)
+    Ky: int = 6
+    Kz: int = 8        # kept non-negative modes along z
     Np: int = 80       # transform length (zero padding beyond T)
     C: int = 32        # width
     L: int = 4         # Fourier layers

@@ -77,8 +77,9 @@ def _gather(a, origin, fill):
     return out
 
 
-def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0):
-    """Collimated beam intensity (relative to its peak) over the window."""
+def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0, with_tau=False):
+    """Collimated beam intensity (relative to its peak) over the window, and
+    with_tau: the optical depth along the beam from the window top."""
     Nx, Ny, Nz = mu_a.shape
     dx, dy, dz = spacing
     x0, y0, z0 = origin
@@ -94,7 +95,9 @@ def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_am
     inx = (X >= 0) & (X < Nx)
     iny = (Y >= 0) & (Y < Ny)
     nprev = np.full(Bm.shape, n_amb)
+    tau = np.zeros(Bm.shape)
     out = np.zeros((SW, SW, SW))
+    tout = np.zeros((SW, SW, SW))
     for k in range(SW):
         Z = z0 + k
         ma = np.zeros(Bm.shape)
@@ -109,10 +112,13 @@ def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_am
             nk = np.where(m, n[xi, yi, Z], n_amb)
         # up-beam neighbour (row index t - sdir), own value at the row ends
         Bn = np.roll(Bm, sdir, axis=0)
+        Tn = np.roll(tau, sdir, axis=0)
         if sdir > 0:
             Bn[0] = Bm[0]
+            Tn[0] = tau[0]
         else:
             Bn[-1] = Bm[-1]
+            Tn[-1] = tau[-1]
         sk = s0 / nk
         ok = sk < 1.0
         ck = np.sqrt(np.clip(1.0 - sk * sk, 0.0, None))
@@ -122,10 +128,12 @@ def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_am
         cp = np.sqrt(np.clip(1.0 - sp * sp, 0.0, None))
         R = fresnel_u(nprev, nk, cp)
         Bk = np.where(ok, bi * (1.0 - R) * np.exp(-(ma + ms) * dz / np.where(ok, ck, 1.0)), 0.0)
+        tau = np.where(ok, (1.0 - sh) * tau + sh * Tn + (ma + ms) * dz / np.where(ok, ck, 1.0), tau + (ma + ms) * dz)
         Bm = Bk
         nprev = nk
         out[:, :, k] = Bm[SW:, :]
-    return out
+        tout[:, :, k] = tau[SW:, :]
+    return (out, tout) if with_tau else out
 
 
 def features(mu_a, mu_s, g, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0):
@@ -135,7 +143,7 @@ def features(mu_a, mu_s, g, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_
     ms = _gather(mu_s, origin, 0.0)
     gg = _gather(g, origin, 0.0)
     nn = _gather(n, origin, n_amb)
-    B = ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi, n_amb)
+    B, tau = ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi, n_amb, with_tau=True)
     x[..., 0] = np.log1p(ma)
     x[..., 1] = np.log1p(ms * 0.1)
     x[..., 2] = gg
@@ -143,7 +151,7 @@ def features(mu_a, mu_s, g, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_
     x[..., 4] = np.log(np.maximum(B, 1e-6)) / LN1E6 + 1.0
     x[..., 5] = math.sin(theta)
     x[..., 6] = spot / 0.5 - 1.0
-    x[..., 7] = np.arange(SW)[None, None, :] / 63.0
+    x[..., 7] = np.log1p(tau) / math.log(101.0)
     return x
 
 

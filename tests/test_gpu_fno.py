@@ -49,7 +49,7 @@ def _rel(a, b):
 @pytest.fixture(scope="module")
 def net():
     from surrogate.network import FourierSurrogate
-    n = FourierSurrogate(max_batch=2, train=True)
+    n = FourierSurrogate(modes=(SPEC.Kx, SPEC.Ky, SPEC.Kz), np_len=SPEC.Np, max_batch=2, train=True)
     assert n.n_params == F.n_params(SPEC)
     yield n
     n.close()
