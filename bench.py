@@ -134,7 +134,17 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each plain epoch as a captured HIP graph")
     ap.add_argument("--plain-only", action="store_true", help="time the plain MC loop only")
     ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--copy-kernel", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_kernel:
+        from simulation import _bdlib as B
+        n = 256 * 1024 * 1024
+        src, dst = B.DeviceArray((n,), dtype="u1"), B.DeviceArray((n,), dtype="u1")
+        for _ in range(3):
+            dst.copy_from(src)
+        print(json.dumps({"copy_bytes_each_way": n, "copies": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
                                    "(the multi-GPU sharding is described in DESIGN.md)", "n_gpus": args.gpus}))

@@ -46,6 +46,7 @@ __device__ __forceinline__ float hmean(float a, float b) { return 2.0f * a * b /
 template <int BY>
 __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
 {
+    static_assert(BY >= 2, "the y-halo rows are loaded by two different waves");
     const int lane = threadIdx.x;
     const int wy = threadIdx.y;
     // XCD-aware block -> tile bijection (blocks are dealt round-robin to 8 XCDs)
@@ -85,37 +86,82 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
     const int zh_lo = tz_i * 64 - 1, zh_hi = tz_i * 64 + 64;
     double econd = 0.0;
     int buf = 0;
+    // per-cell inputs of the next plane, loaded one iteration ahead so their
+    // latency overlaps the current plane's stencil
+    float cap_n = 1.f, q_n = 0.f, h_n = 0.f, chi_n = 0.f;
+    uint8_t fl_n = 0;
+    if (valid && x0 < x1) {
+        const int64_t id0 = x0 * plane + col;
+        cap_n = a.rc[id0];
+        q_n = a.Q[id0];
+        fl_n = a.flags[id0];
+        if (a.plateau && (fl_n & F_HPOS)) h_n = a.H[id0];
+        if (a.chr && (fl_n & F_DRY)) chi_n = a.chi[id0];
+    }
+    // halo cells of a plane (y rows by the edge waves, z ends by the edge lanes),
+    // also loaded one plane ahead
+    const bool hy_lo = wy == 0 && yh_lo >= 0 && z < Nz, hy_hi = wy == BY - 1 && yh_hi < Ny && z < Nz;
+    const bool hz_lo = lane == 0 && zh_lo >= 0 && y < Ny, hz_hi = lane == 63 && zh_hi < Nz && y < Ny;
+    const int64_t off_y = hy_lo ? (int64_t)yh_lo * Nz + z : (int64_t)yh_hi * Nz + z;
+    const int64_t off_z = hz_lo ? (int64_t)y * Nz + zh_lo : (int64_t)y * Nz + zh_hi;
+    float hyT = 0.f, hyK = 0.f, hzT = 0.f, hzK = 0.f;
+    if (x0 < x1) {
+        if (hy_lo || hy_hi) {
+            hyT = a.Tin[x0 * plane + off_y];
+            hyK = a.k[x0 * plane + off_y];
+        }
+        if (hz_lo || hz_hi) {
+            hzT = a.Tin[x0 * plane + off_z];
+            hzK = a.k[x0 * plane + off_z];
+        }
+    }
     for (int x = x0; x < x1; ++x) {
         const int64_t pbase = x * plane;
         sT[buf][wy + 1][lane + 1] = Tc;
         sK[buf][wy + 1][lane + 1] = kc;
-        if (wy == 0 && yh_lo >= 0 && z < Nz) {
-            sT[buf][0][lane + 1] = a.Tin[pbase + (int64_t)yh_lo * Nz + z];
-            sK[buf][0][lane + 1] = a.k[pbase + (int64_t)yh_lo * Nz + z];
+        if (hy_lo) {
+            sT[buf][0][lane + 1] = hyT;
+            sK[buf][0][lane + 1] = hyK;
         }
-        if (wy == BY - 1 && yh_hi < Ny && z < Nz) {
-            sT[buf][BY + 1][lane + 1] = a.Tin[pbase + (int64_t)yh_hi * Nz + z];
-            sK[buf][BY + 1][lane + 1] = a.k[pbase + (int64_t)yh_hi * Nz + z];
+        if (hy_hi) {
+            sT[buf][BY + 1][lane + 1] = hyT;
+            sK[buf][BY + 1][lane + 1] = hyK;
         }
-        if (lane == 0 && zh_lo >= 0 && y < Ny) {
-            sT[buf][wy + 1][0] = a.Tin[pbase + (int64_t)y * Nz + zh_lo];
-            sK[buf][wy + 1][0] = a.k[pbase + (int64_t)y * Nz + zh_lo];
+        if (hz_lo) {
+            sT[buf][wy + 1][0] = hzT;
+            sK[buf][wy + 1][0] = hzK;
         }
-        if (lane == 63 && zh_hi < Nz && y < Ny) {
-            sT[buf][wy + 1][65] = a.Tin[pbase + (int64_t)y * Nz + zh_hi];
-            sK[buf][wy + 1][65] = a.k[pbase + (int64_t)y * Nz + zh_hi];
+        if (hz_hi) {
+            sT[buf][wy + 1][65] = hzT;
+            sK[buf][wy + 1][65] = hzK;
         }
-        float Tn2 = 0.f, kn2 = 0.f, cap = 1.f, qv = 0.f;
-        uint8_t fl = 0;
+        if (x + 1 < x1) {
+            if (hy_lo || hy_hi) {
+                hyT = a.Tin[pbase + plane + off_y];
+                hyK = a.k[pbase + plane + off_y];
+            }
+            if (hz_lo || hz_hi) {
+                hzT = a.Tin[pbase + plane + off_z];
+                hzK = a.k[pbase + plane + off_z];
+            }
+        }
+        float Tn2 = 0.f, kn2 = 0.f;
+        const float cap = cap_n, qv = q_n, h0 = h_n, chi0 = chi_n;
+        const uint8_t fl = fl_n;
         const int64_t id = pbase + col;
         if (valid) {
             if (x + 2 < Nx) {
                 Tn2 = a.Tin[id + 2 * plane];
                 kn2 = a.k[id + 2 * plane];
             }
-            cap = a.rc[id];
-            qv = a.Q[id];
-            fl = a.flags[id];
+            if (x + 1 < x1) {
+                const int64_t idn = id + plane;
+                cap_n = a.rc[idn];
+                q_n = a.Q[idn];
+                fl_n = a.flags[idn];
+                h_n = (a.plateau && (fl_n & F_HPOS)) ? a.H[idn] : 0.f;
+                chi_n = (a.chr && (fl_n & F_DRY)) ? a.chi[idn] : 0.f;
+            }
         }
         __syncthreads();
         if (valid) {
@@ -147,13 +193,13 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
                     if (Tn > a.t_void) Tn = a.t_void;
                 } else if (fl & F_DRY) {
                 } else if (Tn > a.t_abl) {
-                    float h = (fl & F_HPOS) ? a.H[id] : 0.f;
+                    float h = (fl & F_HPOS) ? h0 : 0.f;
                     h += (Tn - a.t_abl) * cap;
                     a.H[id] = h;
                     Tn = a.t_abl;
                     fl2 = h > 0.f ? (uint8_t)(fl | F_HPOS) : (uint8_t)(fl & ~F_HPOS);
                 } else if (fl & F_HPOS) {
-                    float h = a.H[id];
+                    float h = h0;
                     const float rel = fminf(h, (a.t_abl - Tn) * cap);
                     h -= rel;
                     a.H[id] = h;
@@ -164,7 +210,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
             if (a.arr && !(fl & F_ABL)) a.Om[id] += a.dt * a.arrA * expf(-a.arrEaR / (Tn + 273.15f));
             if (a.chr && (fl & F_DRY) && !(fl & F_ABL)) {
                 const float rate = a.chA * expf(-a.chEaR / (Tn + 273.15f));
-                a.chi[id] = 1.0f - (1.0f - a.chi[id]) * expf(-rate * a.dt);
+                a.chi[id] = 1.0f - (1.0f - chi0) * expf(-rate * a.dt);
             }
             a.Tout[id] = Tn;
             if (a.peak) a.Tpeak[id] = fmaxf(a.Tpeak[id], Tn);
@@ -201,7 +247,6 @@ static int launch_one(Ctx *c, SubArgs &a, int by)
     const int64_t tiles = (int64_t)ntz * nty * ntx;
     if (tiles > 0x7fffffff) { set_error("sub-step grid too large"); return BD_EINVAL; }
     switch (by) {
-    case 1: hipLaunchKernelGGL(k_substep<1>, dim3((unsigned)tiles), dim3(64, 1), 0, c->stream, a); break;
     case 2: hipLaunchKernelGGL(k_substep<2>, dim3((unsigned)tiles), dim3(64, 2), 0, c->stream, a); break;
     case 8: hipLaunchKernelGGL(k_substep<8>, dim3((unsigned)tiles), dim3(64, 8), 0, c->stream, a); break;
     default: hipLaunchKernelGGL(k_substep<4>, dim3((unsigned)tiles), dim3(64, 4), 0, c->stream, a); break;

@@ -36,6 +36,7 @@ class SurrogateCoupledEngine:
         self.y = DeviceArray((1, T, T, T))
         self.fallback = False
         self.have_stale = False
+        self.judged = False
         self.history = []
 
     def _predict(self, beam, into_q):
@@ -53,7 +54,9 @@ class SurrogateCoupledEngine:
             d = eng.step_epoch()
             d['optics_source'] = 'none'
             return d
-        judge = (not self.have_stale) or self.fallback or (eng.epoch % self.judge_every == 0)
+        # MC on the first epoch (it also supplies the field outside the window), MC and a
+        # judgement on the next one, then every judge_every-th epoch and while falling back
+        judge = (not self.judged) or self.fallback or (eng.epoch % self.judge_every == 0)
         t0 = time.perf_counter()
         if judge:
             err = None
@@ -65,6 +68,7 @@ class SurrogateCoupledEngine:
                 err = eng._ctx.sur_error()
                 e = err[1] if self.metric == "block4" else err[0]
                 self.fallback = e > self.tol
+                self.judged = True
             eng._ctx.sur_keep_stale()
             self.have_stale = True
             d['optics_source'] = 'mc'

@@ -444,6 +444,16 @@ class OraclePlume:
                       *self.shape, self.dx, self.dy, self.dz, dt_sub, self.D_xy, self.D_z, self.v_drift, decay)
         self.raw_field[:] = self._tmp_field
 
+    def set_wavelength(self, wavelength_nm):
+        """plume.py:270-278."""
+        self.wavelength = float(wavelength_nm)
+        cfg = self.EXTINCTION_MAP.get(int(round(self.wavelength)), {'mu_a_max': 0.30, 'mu_s_max': 0.15,
+                                                                      'delta_mu': 0.15})
+        self.mu_a_max = float(cfg.get('mu_a_max', 0.30))
+        self.mu_s_max = float(cfg.get('mu_s_max', 0.25 * self.mu_a_max))
+        self.mu_max = self.mu_a_max + self.mu_s_max
+        self.delta_mu = float(cfg.get('delta_mu', 0.15))
+
     def sync_extinction(self, void_mask=None):
         """plume.py:352-366."""
         if void_mask is not None:

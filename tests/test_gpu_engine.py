@@ -249,3 +249,54 @@ def test_graph_epochs_match_eager_epochs():
     assert float(np.max(np.abs(T1 - T2))) < 1e-2 * float(T1.max() - 20.0) + 1e-3
     np.testing.assert_allclose(e1.ablation_time[e1.ablated & e2.ablated], e2.ablation_time[e1.ablated & e2.ablated])
     assert e1.t == e2.t and e1.epoch == e2.epoch == 8
+
+
+def _switch(engine, geo2, wl2, p2):
+    """The reference's mid-run wavelength switch, verbatim in effect
+    (run_dual_permutation_suite.py:257-283): intact-voxel optics and dry-state
+    optics replaced in place, wet copies re-taken, dry/char arrays rebuilt,
+    plume re-tuned and re-synced, new wavelength and power."""
+    intact = ~np.asarray(engine.ablated)
+    for key in ('mu_a', 'mu_s', 'g', 'n'):
+        engine.optical[key][intact] = geo2['optical'][key][intact]
+        engine.dry_state[key][intact] = geo2['dry_state'][key][intact]
+    engine._wet = (engine.optical['mu_a'].copy(), engine.k_grid.copy(), engine.rho_cp_grid.copy())
+    engine._ds_arrays = None
+    engine._init_ds_cs_arrays()
+    if engine.plume is not None:
+        engine.plume.set_wavelength(wl2)
+        engine.plume.sync_extinction(engine.compute_void_mask())
+    engine.wavelength_nm = float(wl2)
+    engine.power = p2
+
+
+def test_mid_run_wavelength_switch_matches_oracle(oracle):
+    """Three epochs at 532 nm / 30 W, the reference's switch sequence, three
+    epochs at 1470 nm / 60 W: device engine against the FP64 oracle engine."""
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.workloads import layered_stack
+    kw1 = layered_stack(shape=(32, 32, 56), n_photons=40_000, plume_grid=None, seed=3, power=30.0)
+    kw2 = layered_stack(shape=(32, 32, 56), n_photons=40_000, plume_grid=None, seed=5, power=60.0)
+    geo2 = {'optical': kw2['optical_props'], 'dry_state': kw2['dry_state']}
+
+    def fresh(kw):
+        return {k: ({kk: (vv.copy() if isinstance(vv, np.ndarray) else vv) for kk, vv in v.items()}
+                    if isinstance(v, dict) else (v.copy() if isinstance(v, np.ndarray) else v)) for k, v in kw.items()}
+    eng = CoupledAblationEngine(**fresh(kw1))
+    k_or = fresh(kw1)
+    k_or.pop('plume_grid')
+    ref = oracle.OracleEngine(mc_threads=8, **k_or)
+    for ep in range(6):
+        if ep == 3:
+            _switch(eng, geo2, 1470.0, 60.0)
+            _switch(ref, geo2, 1470.0, 60.0)
+            intact = ~np.asarray(eng.ablated)
+            np.testing.assert_allclose(np.asarray(eng.optical['mu_a'])[intact], geo2['optical']['mu_a'][intact],
+                                       rtol=1e-7)
+        a, b = eng.step_epoch(), ref.step_epoch()
+        assert abs(a['absorbed_frac'] - b['absorbed_frac']) < 0.01, ep
+        assert abs(a['plume_transmission'] - b['plume_transmission']) < 0.02, ep
+    Tg, Tr = np.asarray(eng.T), ref.T
+    rise = float(Tr.max() - 20.0)
+    assert abs(float(Tg.max()) - float(Tr.max())) < 0.05 * rise
+    assert eng.plume.wavelength == 1470.0 and eng.wavelength_nm == 1470.0

@@ -59,9 +59,11 @@ def test_loop_judges_every_tenth_epoch_and_falls_back():
     for _ in range(11):
         sur.step_epoch()
     src = [h["optics_source"] for h in sur.history]
-    assert src == ["mc"] + ["surrogate"] * 4 + ["mc"] + ["surrogate"] * 4 + ["mc"]
+    # epoch 0: MC (the field outside the window); 1: MC + first judgement; then every 5th
+    assert src == ["mc", "mc"] + ["surrogate"] * 3 + ["mc"] + ["surrogate"] * 4 + ["mc"]
     errs = [h["judge_error"] for h in sur.history if h["judge_error"] is not None]
-    assert len(errs) == 2 and all(0.0 <= e[1] < 10.0 for e in errs)
+    assert len(errs) == 3 and all(0.0 <= e[1] < 10.0 for e in errs)
+    print("judged (voxel, block4):", errs)
     sur.close()
     eng.close()
     eng = _engine()
