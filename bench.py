@@ -88,7 +88,7 @@ def run_plain(kw, steps, warmup, graph):
             log(f"bench: plain epoch {i + 1}/{steps}")
     eng._ctx.sync()
     wall = time.perf_counter() - t0
-    out = {"wall": wall, "events": events, "nnew": nnew, "nvox": eng.Nx * eng.Ny * eng.Nz,
+    out = {"wall": wall, "events": events, "nnew": nnew, "nvox": eng.Nx * eng.Ny * eng.Nz, "tuning": eng._ctx.tuning(),
            "n_sub": eng.n_substeps, "dt_epoch": eng.dt_epoch, "n_photons": eng.n_photons,
            **{k: np.asarray(v) for k, v in rec.items()}, "state": final_state(eng, 21)}
     eng.close()
@@ -159,8 +159,14 @@ def main():
     kw = layered_stack()
     p = run_plain(kw, args.steps, args.warmup, args.graph)
     plain_rate = args.steps * p["dt_epoch"] * 1e3 / p["wall"]
-    sub_launch_ms = float(np.mean(p["sub"])) / p["n_sub"]
+    # a launch of the pair kernel advances two sub-steps and moves each field once,
+    # so its algorithmic bytes per voxel are those of one sub-step
+    tun = p["tuning"]
+    n_pair = 2 * (p["n_sub"] // 4) if tun["pairs"] else 0
+    n_launch = n_pair + (p["n_sub"] - 2 * n_pair)
+    sub_launch_ms = float(np.mean(p["sub"])) / n_launch
     achieved = SUBSTEP_BYTES_PER_VOXEL * p["nvox"] / (sub_launch_ms * 1e-3) / 1e9
+    per_substep_ms = float(np.mean(p["sub"])) / p["n_sub"]
     mc_avg = float(np.mean(p["mc"]))
     s = None
     if not args.plain_only:
@@ -199,7 +205,8 @@ def main():
                          "substeps": round(float(np.mean(p["sub"])), 4), "mc_kernel": round(mc_avg, 4),
                          "phase_plume": round(float(np.mean(p["ph"])), 4)},
         "roofline": {
-            "kernel": "k_substep (fused bioheat sub-step)",
+            "kernel": "k_substep2 (two fused bioheat sub-steps per launch)" if n_pair else
+                      "k_substep (fused bioheat sub-step)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -208,6 +215,9 @@ def main():
             "traffic": None,
             "bytes_per_voxel": SUBSTEP_BYTES_PER_VOXEL,
             "avg_launch_ms": round(sub_launch_ms, 5),
+            "launches_per_epoch": n_launch,
+            "ms_per_substep": round(per_substep_ms, 5),
+            "tuning": tun,
         },
     }
     if s:

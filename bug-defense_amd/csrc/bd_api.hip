@@ -143,6 +143,7 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     BD_HIP(hipEventCreate(&c->ev5));
     for (int i = 0; i < 6; ++i) BD_HIP(hipEventCreate(&c->gev[i]));
     if (const char *e = std::getenv("BUGDEF_MC_BLOCKS_PER_CU")) c->mc_blocks_per_cu = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("BUGDEF_SUBSTEP_PAIRS")) c->two_step_allowed = std::atoi(e) != 0;
     BD_HIP(hipHostMalloc(&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
     BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + 1024)));
     BD_TRY(ensure(c, (void **)&c->counter, sizeof(unsigned long long) * 16));
@@ -158,7 +159,7 @@ int bd_destroy(bd_ctx *h)
     void *bufs[] = {c->T, c->T2, c->H, c->Om, c->chi, c->Q, c->k, c->rc, c->hlat, c->flags, c->mu_a, c->mu_s,
                     c->gg, c->nn, c->wet_mu_a, c->wet_k, c->wet_rc, c->Tpeak, c->abl_time, c->AL, c->optics,
                     c->P, c->P2, c->Pext, c->pvoid, c->to_remove, c->red, c->lat, c->counter, c->staging,
-                    c->splat_tmp1, c->splat_tmp2, c->kb[0], c->kb[1], c->kb[2], c->kl[0], c->kl[1], c->kl[2], c->Qs, c->Qsur};
+                    c->splat_tmp1, c->splat_tmp2, c->kb[0], c->kb[1], c->kb[2], c->kl[0], c->kl[1], c->kl[2], c->Qs, c->Qsur, c->Hb, c->flagsb};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &f : c->ds)
@@ -707,6 +708,20 @@ int bd_sync(bd_ctx *h)
 {
     BD_CHECK(h, "bd_sync: null argument");
     BD_HIP(hipStreamSynchronize(h->c.stream));
+    return BD_OK;
+}
+
+int bd_get_tuning(bd_ctx *h, int32_t *out, int n)
+{
+    BD_CHECK(h && out, "bd_get_tuning: null argument");
+    BD_CHECK(n >= 6, "bd_get_tuning: out must hold 6 values");
+    const Ctx *c = &h->c;
+    out[0] = c->tile_by;
+    out[1] = c->tile_xc;
+    out[2] = c->tile2_by;
+    out[3] = c->tile2_xc;
+    out[4] = c->use_two ? 1 : 0;
+    out[5] = c->tuned ? 1 : 0;
     return BD_OK;
 }
 

@@ -233,7 +233,8 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
     BD_TRY(ensure(c, (void **)&c->to_remove, (size_t)c->nvox));
     if (c->phys.has_arrhenius) BD_TRY(ensure(c, (void **)&c->Om, (size_t)c->nvox * 4));
     const bool recapture = !c->graph_exec || c->graph_nsub != n_substeps || !same_beam(c->graph_beam, *beam) ||
-                           c->graph_T != c->T || c->graph_P != c->P;
+                           c->graph_T != c->T || c->graph_P != c->P || c->graph_H != c->H ||
+                           c->graph_F != c->flags;
     if (recapture) {
         // allocate every lazily created buffer and tune the tiles before capture
         BD_TRY(launch_substeps(c, 0, 1, BD_SUB_PLATEAU, true));
@@ -260,6 +261,8 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
         BD_HIP(hipStreamSynchronize(c->stream));
         c->graph_T = c->T;
         c->graph_P = c->P;
+        c->graph_H = c->H;
+        c->graph_F = c->flags;
         BD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const int rc = enqueue_epoch(c, beam, n_substeps);
         hipGraph_t g = nullptr;

@@ -95,8 +95,13 @@ struct Ctx {
     // persistent MC grid: blocks of 256 per CU (0: one lane per photon up to 8 blocks per CU)
     int mc_blocks_per_cu = 0;
 
-    // tuned launch shape of the sub-step kernel
+    // tuned launch shape of the sub-step kernel, and of the two-sub-step kernel
     int tile_by = 4, tile_xc = 16;
+    int tile2_by = 16, tile2_xc = 32;
+    bool use_two = false;
+    bool two_step_allowed = true;      // BUGDEF_SUBSTEP_PAIRS=0 keeps every sub-step in its own launch
+    float *Hb = nullptr;               // second H_acc buffer (the two-sub-step kernel writes the other one)
+    uint8_t *flagsb = nullptr;         // second flags buffer
     bool tuned = false;
     double last_ledger_conducted = 0.0;
 
@@ -107,6 +112,8 @@ struct Ctx {
     bd_beam graph_beam{};
     int graph_nsub = -1;
     const float *graph_T = nullptr, *graph_P = nullptr;  // ping-pong heads the graph was captured with
+    const float *graph_H = nullptr;
+    const uint8_t *graph_F = nullptr;
     bool graph_pending = false;          // launched by bd_epoch_graph_launch, not yet finished
     bd_beam pending_beam{};
     double pending_dt_epoch = 0.0;

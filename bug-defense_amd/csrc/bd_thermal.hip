@@ -17,6 +17,7 @@
 #include "bd_internal.hpp"
 
 #include <cmath>
+#include <utility>
 
 namespace bd {
 
@@ -39,14 +40,114 @@ struct SubArgs {
     float arrA, arrEaR, chA, chEaR;
     int plateau, arr, chr, maskq, peak;
     double vol_dt;
+    // two-sub-step kernel: H and flags are read from one buffer and written to another
+    const uint8_t *__restrict__ flags_in;
+    const float *__restrict__ H_in;
 };
 
+
 __device__ __forceinline__ float hmean(float a, float b) { return 2.0f * a * b / (a + b + 1e-12f); }
+
+// Face conductance and 1/rho_cp with the hardware reciprocal (v_rcp_f32, 1 ulp):
+// the stencil's divisions dominate its ALU time, and the reference evaluates
+// this loop under fastmath (voxel_thermal_3d.py:30, numba fastmath=True).
+__device__ __forceinline__ float hmean_fast(float a, float b)
+{
+    return 2.0f * a * b * __builtin_amdgcn_rcpf(a + b + 1e-12f);
+}
+
+// Constants of one sub-step, shared by the one- and two-step kernels.
+struct StepK {
+    int Nx, Ny, Nz;
+    float idx2, idy2, idz2, dz, inv_h, h_eff;
+    float dt, t_amb, t_abl, t_void, qscale;
+    float arrA, arrEaR, chA, chEaR;
+    int plateau, arr, chr, maskq;
+};
+
+__device__ __forceinline__ StepK step_consts(const SubArgs &a)
+{
+    StepK k;
+    k.Nx = a.Nx; k.Ny = a.Ny; k.Nz = a.Nz;
+    k.idx2 = a.idx2; k.idy2 = a.idy2; k.idz2 = a.idz2; k.dz = a.dz; k.inv_h = a.inv_h; k.h_eff = a.h_eff;
+    k.dt = a.dt; k.t_amb = a.t_amb; k.t_abl = a.t_abl; k.t_void = a.t_void; k.qscale = a.qscale;
+    k.arrA = a.arrA; k.arrEaR = a.arrEaR; k.chA = a.chA; k.chEaR = a.chEaR;
+    k.plateau = a.plateau; k.arr = a.arr; k.chr = a.chr; k.maskq = a.maskq;
+    return k;
+}
+
+// One explicit sub-step of one cell (voxel_thermal_3d.py:30-143): face fluxes in
+// the reference's order (w, e, s, n, top, bottom), source, the latent-heat
+// plateau on (h, fl).  Both kernels evaluate every cell through this function
+// with contraction off, so the two paths produce the same bits.  Returns the
+// new T; *fbound gets the boundary flux of the cell.
+__device__ __forceinline__ float cell_step(const StepK &a, int x, int y, int z, float Tc, float kc, float Tm, float km,
+                                           float Tp, float kp, float Ts, float ks, float Tn_, float kn, float Tt,
+                                           float kt, float Tb, float kb, float cap, float qv, uint8_t fl, float &h,
+                                           uint8_t &fl2, float *fbound)
+{
+#pragma clang fp contract(off)
+    float fw = 0.f, fe = 0.f, fs = 0.f, fn = 0.f, ft = 0.f, fb = 0.f;
+    if (x > 0) fw = hmean_fast(kc, km) * (Tm - Tc) * a.idx2;
+    if (x < a.Nx - 1) fe = hmean_fast(kc, kp) * (Tp - Tc) * a.idx2;
+    if (y > 0) fs = hmean_fast(kc, ks) * (Ts - Tc) * a.idy2;
+    if (y < a.Ny - 1) fn = hmean_fast(kc, kn) * (Tn_ - Tc) * a.idy2;
+    float fbd = 0.f;
+    if (z > 0) ft = hmean_fast(kc, kt) * (Tt - Tc) * a.idz2;
+    else if (a.h_eff > 0.f) {
+        ft = (a.t_amb - Tc) / (a.dz * (0.5f * a.dz / fmaxf(kc, 1e-30f) + a.inv_h));
+        fbd += ft;
+    }
+    if (z < a.Nz - 1) fb = hmean_fast(kc, kb) * (Tb - Tc) * a.idz2;
+    else if (a.h_eff > 0.f) {
+        fb = (a.t_amb - Tc) / (a.dz * (0.5f * a.dz / fmaxf(kc, 1e-30f) + a.inv_h));
+        fbd += fb;
+    }
+    const float div = fw + fe + fs + fn + ft + fb;
+    float src = qv * a.qscale;
+    if (a.maskq && (fl & F_ABL)) src = 0.f;
+    float Tn = Tc + a.dt * (div + src) * __builtin_amdgcn_rcpf(cap);
+    *fbound = fbd;
+    fl2 = fl;
+    if (a.plateau) {
+        if (fl & F_ABL) {
+            if (Tn > a.t_void) Tn = a.t_void;
+        } else if (fl & F_DRY) {
+        } else if (Tn > a.t_abl) {
+            float hh = (fl & F_HPOS) ? h : 0.f;
+            hh += (Tn - a.t_abl) * cap;
+            h = hh;
+            Tn = a.t_abl;
+            fl2 = hh > 0.f ? (uint8_t)(fl | F_HPOS) : (uint8_t)(fl & ~F_HPOS);
+        } else if (fl & F_HPOS) {
+            float hh = h;
+            const float rel = fminf(hh, (a.t_abl - Tn) * cap);
+            hh -= rel;
+            h = hh;
+            Tn = Tn + rel / cap;
+            if (!(hh > 0.f)) fl2 = (uint8_t)(fl & ~F_HPOS);
+        }
+    }
+    return Tn;
+}
+
+// Arrhenius damage and char kinetics of one cell at its new temperature
+// (coupled_ablation.py:33-58)
+__device__ __forceinline__ void cell_damage(const StepK &a, float Tn, uint8_t fl, float &om, float &chi)
+{
+#pragma clang fp contract(off)
+    if (a.arr && !(fl & F_ABL)) om += a.dt * a.arrA * expf(-a.arrEaR / (Tn + 273.15f));
+    if (a.chr && (fl & F_DRY) && !(fl & F_ABL)) {
+        const float rate = a.chA * expf(-a.chEaR / (Tn + 273.15f));
+        chi = 1.0f - (1.0f - chi) * expf(-rate * a.dt);
+    }
+}
 
 template <int BY>
 __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
 {
     static_assert(BY >= 2, "the y-halo rows are loaded by two different waves");
+    const StepK K = step_consts(a);
     const int lane = threadIdx.x;
     const int wy = threadIdx.y;
     // XCD-aware block -> tile bijection (blocks are dealt round-robin to 8 XCDs)
@@ -165,52 +266,19 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
         }
         __syncthreads();
         if (valid) {
-            // face fluxes in the reference's summation order: w, e, s, n, top, bottom
-            float fw = 0.f, fe = 0.f, fs = 0.f, fn = 0.f, ft = 0.f, fb = 0.f;
-            if (x > 0) fw = hmean(kc, km) * (Tm - Tc) * a.idx2;
-            if (x < Nx - 1) fe = hmean(kc, kp) * (Tp - Tc) * a.idx2;
-            if (y > 0) fs = hmean(kc, sK[buf][wy][lane + 1]) * (sT[buf][wy][lane + 1] - Tc) * a.idy2;
-            if (y < Ny - 1) fn = hmean(kc, sK[buf][wy + 2][lane + 1]) * (sT[buf][wy + 2][lane + 1] - Tc) * a.idy2;
-            float fbound = 0.f;
-            if (z > 0) ft = hmean(kc, sK[buf][wy + 1][lane]) * (sT[buf][wy + 1][lane] - Tc) * a.idz2;
-            else if (a.h_eff > 0.f) {
-                ft = (a.t_amb - Tc) / (a.dz * (0.5f * a.dz / fmaxf(kc, 1e-30f) + a.inv_h));
-                fbound += ft;
-            }
-            if (z < Nz - 1) fb = hmean(kc, sK[buf][wy + 1][lane + 2]) * (sT[buf][wy + 1][lane + 2] - Tc) * a.idz2;
-            else if (a.h_eff > 0.f) {
-                fb = (a.t_amb - Tc) / (a.dz * (0.5f * a.dz / fmaxf(kc, 1e-30f) + a.inv_h));
-                fbound += fb;
-            }
-            const float div = fw + fe + fs + fn + ft + fb;
-            float src = qv * a.qscale;
-            if (a.maskq && (fl & F_ABL)) src = 0.f;
-            float Tn = Tc + a.dt * (div + src) / cap;
+            float h = h0, fbound = 0.f;
+            uint8_t fl2;
+            const float Tn = cell_step(K, x, y, z, Tc, kc, Tm, km, Tp, kp, sT[buf][wy][lane + 1],
+                                       sK[buf][wy][lane + 1], sT[buf][wy + 2][lane + 1], sK[buf][wy + 2][lane + 1],
+                                       sT[buf][wy + 1][lane], sK[buf][wy + 1][lane], sT[buf][wy + 1][lane + 2],
+                                       sK[buf][wy + 1][lane + 2], cap, qv, fl, h, fl2, &fbound);
             econd += (double)fbound;
-            uint8_t fl2 = fl;
-            if (a.plateau) {
-                if (fl & F_ABL) {
-                    if (Tn > a.t_void) Tn = a.t_void;
-                } else if (fl & F_DRY) {
-                } else if (Tn > a.t_abl) {
-                    float h = (fl & F_HPOS) ? h0 : 0.f;
-                    h += (Tn - a.t_abl) * cap;
-                    a.H[id] = h;
-                    Tn = a.t_abl;
-                    fl2 = h > 0.f ? (uint8_t)(fl | F_HPOS) : (uint8_t)(fl & ~F_HPOS);
-                } else if (fl & F_HPOS) {
-                    float h = h0;
-                    const float rel = fminf(h, (a.t_abl - Tn) * cap);
-                    h -= rel;
-                    a.H[id] = h;
-                    Tn = Tn + rel / cap;
-                    if (!(h > 0.f)) fl2 = (uint8_t)(fl & ~F_HPOS);
-                }
-            }
-            if (a.arr && !(fl & F_ABL)) a.Om[id] += a.dt * a.arrA * expf(-a.arrEaR / (Tn + 273.15f));
-            if (a.chr && (fl & F_DRY) && !(fl & F_ABL)) {
-                const float rate = a.chA * expf(-a.chEaR / (Tn + 273.15f));
-                a.chi[id] = 1.0f - (1.0f - chi0) * expf(-rate * a.dt);
+            if (fl2 != fl || h != h0) a.H[id] = h;
+            if (a.arr || a.chr) {
+                float om = a.arr ? a.Om[id] : 0.f, chi = chi0;
+                cell_damage(K, Tn, fl, om, chi);
+                if (a.arr && !(fl & F_ABL)) a.Om[id] = om;
+                if (a.chr && (fl & F_DRY) && !(fl & F_ABL)) a.chi[id] = chi;
             }
             a.Tout[id] = Tn;
             if (a.peak) a.Tpeak[id] = fmaxf(a.Tpeak[id], Tn);
@@ -223,6 +291,159 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
     }
     if (a.ledger) {
         // block reduction of the boundary energy (only boundary cells contribute)
+        __shared__ double red[64 * BY];
+        red[wy * 64 + lane] = econd;
+        __syncthreads();
+        for (int w = 32 * BY; w > 0; w >>= 1) {
+            const int tid = wy * 64 + lane;
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        if (wy == 0 && lane == 0 && red[0] != 0.0) unsafeAtomicAdd(a.ledger, red[0] * a.vol_dt);
+    }
+}
+
+// Two sub-steps per launch (temporal blocking).  A block owns a 60(z) x
+// (BY-4)(y) tile and marches along x; its 64 lanes x BY rows also cover a
+// two-cell halo ring.  Iteration p computes sub-step 1 on plane p for the tile
+// and the inner halo ring (from the input T in LDS and registers) and sub-step
+// 2 on plane p-1 for the owned cells (from the sub-step-1 values of the
+// previous iteration, held in LDS and registers), so the intermediate field
+// never goes to HBM: per two sub-steps each field is read once and T, H and
+// the flags are written once.  Other blocks still read the input H and flags
+// of their halo cells, so those two are read from one buffer and written to
+// another (the caller swaps them); chi and Omega only matter to their owner
+// and stay in place.  Every cell goes through cell_step, so the result is the
+// same as two launches of k_substep.
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void k_substep2(SubArgs a)
+{
+    static_assert(BY >= 6, "two halo rows on each side");
+    constexpr int IY = BY - 4, IZ = 60;
+    const StepK K = step_consts(a);
+    const int lane = threadIdx.x, wy = threadIdx.y;
+    const int G = gridDim.x, b = blockIdx.x;
+    const int xcd = b & 7, local = b >> 3, q8 = G >> 3, r8 = G & 7;
+    const int t = xcd * q8 + (xcd < r8 ? xcd : r8) + local;
+    const int tx_i = t % a.ntx;
+    const int rest = t / a.ntx;
+    const int tz_i = rest % a.ntz;
+    const int ty_i = rest / a.ntz;
+    const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
+    const int z = tz_i * IZ - 2 + lane;
+    const int y = ty_i * IY - 2 + wy;
+    const bool in = z >= 0 && z < Nz && y >= 0 && y < Ny;
+    const bool ring = in && lane >= 1 && lane <= 62 && wy >= 1 && wy <= BY - 2;   // sub-step 1 cells
+    const bool own = in && lane >= 2 && lane <= 61 && wy >= 2 && wy <= BY - 3;    // sub-step 2 (written) cells
+    const int x0 = tx_i * a.xc;
+    const int x1 = min(x0 + a.xc, Nx);
+    const int64_t plane = (int64_t)Ny * Nz;
+    const int64_t col = in ? (int64_t)y * Nz + z : 0;
+
+    // sK holds the k plane read by sub-step 1 of this iteration and by sub-step 2 of
+    // the next one, so it needs three slots: the slot written before this
+    // iteration's barrier was last read by sub-step 2 two iterations ago
+    __shared__ float sT0[2][BY][64], sK[3][BY][64], sT1[2][BY][64];
+
+    auto ldT = [&](int p) { return (in && p >= 0 && p < Nx) ? a.Tin[p * plane + col] : 0.f; };
+    auto ldK = [&](int p) { return (in && p >= 0 && p < Nx) ? a.k[p * plane + col] : 0.f; };
+    struct Cell {
+        float cap, q, h, chi, om;
+        uint8_t fl;
+    };
+    auto ldCell = [&](int p) {
+        Cell c{1.f, 0.f, 0.f, 0.f, 0.f, 0};
+        if (ring && p >= 0 && p < Nx) {
+            const int64_t id = p * plane + col;
+            c.cap = a.rc[id];
+            c.q = a.Q[id];
+            c.fl = a.flags_in[id];
+            if (a.plateau && (c.fl & F_HPOS)) c.h = a.H_in[id];
+            if (own && p >= x0 && p < x1) {
+                if (a.chr && (c.fl & F_DRY)) c.chi = a.chi[id];
+                if (a.arr) c.om = a.Om[id];
+            }
+        }
+        return c;
+    };
+
+    float T0m = ldT(x0 - 2), T0c = ldT(x0 - 1), T0p = ldT(x0);
+    float Kmm = ldK(x0 - 3), Km = ldK(x0 - 2), Kc = ldK(x0 - 1), Kp = ldK(x0);
+    float T1m = 0.f, T1c = 0.f;
+    Cell cp = ldCell(x0 - 1), cq{1.f, 0.f, 0.f, 0.f, 0.f, 0};
+    float h1q = 0.f, chi1q = 0.f, om1q = 0.f;
+    uint8_t fl1q = 0;
+    double econd = 0.0;
+    int buf = 0, kb = 0, kbp = 2;  // sK slot of plane p and of plane p - 1
+    for (int p = x0 - 1; p <= x1; ++p) {
+        sT0[buf][wy][lane] = T0c;
+        sK[kb][wy][lane] = Kc;
+        // next iteration's inputs, in flight across the barrier and the two sub-steps
+        const float T0n = ldT(p + 2), Kn = ldK(p + 2);
+        Cell cn{1.f, 0.f, 0.f, 0.f, 0.f, 0};
+        if (p + 1 <= x1) cn = ldCell(p + 1);
+        __syncthreads();
+        // ---- sub-step 1 on plane p (tile + inner ring)
+        float T1p = 0.f, h1p = cp.h, chi1p = cp.chi, om1p = cp.om;
+        uint8_t fl1p = cp.fl;
+        if (ring && p >= 0 && p < Nx) {
+            float fbound = 0.f;
+            T1p = cell_step(K, p, y, z, T0c, Kc, T0m, Km, T0p, Kp, sT0[buf][wy - 1][lane], sK[kb][wy - 1][lane],
+                            sT0[buf][wy + 1][lane], sK[kb][wy + 1][lane], sT0[buf][wy][lane - 1],
+                            sK[kb][wy][lane - 1], sT0[buf][wy][lane + 1], sK[kb][wy][lane + 1], cp.cap, cp.q, cp.fl,
+                            h1p, fl1p, &fbound);
+            if (own && p >= x0 && p < x1) {
+                econd += (double)fbound;
+                cell_damage(K, T1p, cp.fl, om1p, chi1p);
+            }
+        }
+        sT1[buf][wy][lane] = T1p;
+        // ---- sub-step 2 on plane q = p - 1 (owned cells), from the sub-step-1 plane
+        // written in the previous iteration
+        const int q = p - 1;
+        if (own && q >= x0 && q < x1) {
+            const int ob = buf ^ 1;
+            float h2 = h1q, fbound = 0.f;
+            uint8_t fl2;
+            const float T2 = cell_step(K, q, y, z, T1c, Km, T1m, Kmm, T1p, Kc, sT1[ob][wy - 1][lane],
+                                       sK[kbp][wy - 1][lane], sT1[ob][wy + 1][lane], sK[kbp][wy + 1][lane],
+                                       sT1[ob][wy][lane - 1], sK[kbp][wy][lane - 1], sT1[ob][wy][lane + 1],
+                                       sK[kbp][wy][lane + 1], cq.cap, cq.q, fl1q, h2, fl2, &fbound);
+            econd += (double)fbound;
+            float om2 = om1q, chi2 = chi1q;
+            cell_damage(K, T2, fl1q, om2, chi2);
+            const int64_t id = q * plane + col;
+            a.Tout[id] = T2;
+            if (a.peak) a.Tpeak[id] = fmaxf(a.Tpeak[id], T2);
+            // the output H buffer must read 0 wherever its flags say nothing is banked:
+            // write when banked now or when this buffer's previous contents were banked
+            const uint8_t fl_prev_out = a.flags[id];
+            if ((fl2 | fl_prev_out) & F_HPOS) a.H[id] = (fl2 & F_HPOS) ? h2 : 0.f;
+            a.flags[id] = fl2;
+            if (a.arr && !(fl1q & F_ABL)) a.Om[id] = om2;
+            if (a.chr && (fl1q & F_DRY) && !(fl1q & F_ABL)) a.chi[id] = chi2;
+        }
+        // roll the plane registers
+        T0m = T0c;
+        T0c = T0p;
+        T0p = T0n;
+        Kmm = Km;
+        Km = Kc;
+        Kc = Kp;
+        Kp = Kn;
+        T1m = T1c;
+        T1c = T1p;
+        cq = cp;
+        h1q = h1p;
+        fl1q = fl1p;
+        chi1q = chi1p;
+        om1q = om1p;
+        cp = cn;
+        buf ^= 1;
+        kbp = kb;
+        kb = kb == 2 ? 0 : kb + 1;
+    }
+    if (a.ledger) {
         __shared__ double red[64 * BY];
         red[wy * 64 + lane] = econd;
         __syncthreads();
@@ -251,6 +472,23 @@ static int launch_one(Ctx *c, SubArgs &a, int by)
     case 8: hipLaunchKernelGGL(k_substep<8>, dim3((unsigned)tiles), dim3(64, 8), 0, c->stream, a); break;
     default: hipLaunchKernelGGL(k_substep<4>, dim3((unsigned)tiles), dim3(64, 4), 0, c->stream, a); break;
     }
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+static int launch_two(Ctx *c, SubArgs &a, int by, int xc)
+{
+    const int ntz = (c->g.nz + 59) / 60;
+    const int nty = (c->g.ny + (by - 4) - 1) / (by - 4);
+    const int ntx = (c->g.nx + xc - 1) / xc;
+    a.ntz = ntz;
+    a.nty = nty;
+    a.ntx = ntx;
+    a.xc = xc;
+    const int64_t tiles = (int64_t)ntz * nty * ntx;
+    if (tiles > 0x7fffffff) { set_error("sub-step grid too large"); return BD_EINVAL; }
+    if (by == 8) hipLaunchKernelGGL(k_substep2<8>, dim3((unsigned)tiles), dim3(64, 8), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_substep2<16>, dim3((unsigned)tiles), dim3(64, 16), 0, c->stream, a);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
@@ -297,12 +535,21 @@ static int autotune(Ctx *c)
 {
     if (c->tuned) return BD_OK;
     c->tuned = true;
-    if (c->nvox < (1 << 20)) {  // small grids: fixed shape, nothing to win
+    if (c->nvox < (1 << 20)) {  // small grids: fixed shapes, nothing to win
         c->tile_by = 4;
         c->tile_xc = 8;
+        c->tile2_by = 8;
+        c->tile2_xc = 16;
+        c->use_two = c->two_step_allowed;
+        if (c->use_two) {
+            BD_TRY(ensure(c, (void **)&c->Hb, (size_t)c->nvox * 4));
+            BD_TRY(ensure(c, (void **)&c->flagsb, (size_t)c->nvox));
+        }
         return BD_OK;
     }
-    BD_TRY(ensure_staging(c, (size_t)c->nvox * 8));
+    // staging: T copy [0, 4n), H copy [4n, 8n), flags scratch [8n, 9n); sized once, before
+    // any pointer into it is taken
+    BD_TRY(ensure_staging(c, (size_t)c->nvox * 10));
     // scratch copies so the tuning launches leave the state untouched
     float *Tscr = (float *)c->staging;
     float *Hscr = (float *)((char *)c->staging + (size_t)c->nvox * 4);
@@ -340,6 +587,48 @@ static int autotune(Ctx *c)
         }
     c->tile_by = bb;
     c->tile_xc = bx;
+    // two-sub-step kernel: tile shape, and whether it beats two one-step launches
+    uint8_t *Fscr = (uint8_t *)((char *)c->staging + (size_t)c->nvox * 8);
+    float *Hscr2 = (float *)c->T2;  // output H of the tuning launches (T2 is scratch between sub-steps)
+    BD_HIP(hipMemsetAsync(Fscr, 0, (size_t)c->nvox, c->stream));
+    const int bys2[] = {8, 16};
+    const int xcs2[] = {16, 32, 64};
+    float best2 = 1e30f;
+    int bb2 = 16, bx2 = 32;
+    for (int by : bys2)
+        for (int xc : xcs2) {
+            SubArgs a{};
+            fill_args(c, a, 1, BD_SUB_PLATEAU);
+            a.Tin = Tscr;
+            a.Tout = (float *)((char *)c->staging + (size_t)c->nvox * 4);  // Hscr slot reused as T output
+            a.H_in = Hscr;
+            a.H = Hscr2;
+            a.flags_in = c->flags;
+            a.flags = Fscr;
+            a.plateau = 0;
+            a.chr = 0;
+            a.arr = 0;
+            float ms = 0.f;
+            BD_TRY(launch_two(c, a, by, xc));
+            BD_HIP(hipEventRecord(c->ev0, c->stream));
+            for (int r = 0; r < 3; ++r) BD_TRY(launch_two(c, a, by, xc));
+            BD_HIP(hipEventRecord(c->ev1, c->stream));
+            BD_HIP(hipEventSynchronize(c->ev1));
+            BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            if (ms < best2) {
+                best2 = ms;
+                bb2 = by;
+                bx2 = xc;
+            }
+        }
+    c->tile2_by = bb2;
+    c->tile2_xc = bx2;
+    c->use_two = c->two_step_allowed && best2 < 2.f * best;  // three one-step vs three two-step launches
+    if (c->use_two) {
+        // the second H / flags buffers exist before any epoch is captured
+        BD_TRY(ensure(c, (void **)&c->Hb, (size_t)c->nvox * 4));
+        BD_TRY(ensure(c, (void **)&c->flagsb, (size_t)c->nvox));
+    }
     return BD_OK;
 }
 
@@ -353,14 +642,31 @@ int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger)
     a.Om = c->Om;
     a.Tpeak = c->Tpeak;
     a.ledger = ledger ? c->red + 8 : nullptr;  // red[8]: conducted energy of this call
-    for (int s = 0; s < n; ++s) {
+    // pairs of sub-steps in the two-step kernel, an even number of them so the
+    // H / flags buffers end where they started; the rest one at a time
+    const int n_two = c->use_two ? 2 * (n / 4) : 0;
+    int done = 0;
+    for (int s = 0; s < n_two; ++s, done += 2) {
         a.Tin = c->T;
         a.Tout = c->T2;
-        a.peak = ((flags & BD_SUB_PEAK) && s == n - 1) ? 1 : 0;
+        a.H_in = c->H;
+        a.H = c->Hb;
+        a.flags_in = c->flags;
+        a.flags = c->flagsb;
+        a.peak = ((flags & BD_SUB_PEAK) && done + 2 == n) ? 1 : 0;
+        BD_TRY(launch_two(c, a, c->tile2_by, c->tile2_xc));
+        std::swap(c->T, c->T2);
+        std::swap(c->H, c->Hb);
+        std::swap(c->flags, c->flagsb);
+    }
+    a.H = c->H;
+    a.flags = c->flags;
+    for (; done < n; ++done) {
+        a.Tin = c->T;
+        a.Tout = c->T2;
+        a.peak = ((flags & BD_SUB_PEAK) && done == n - 1) ? 1 : 0;
         BD_TRY(launch_one(c, a, c->tile_by));
-        float *tmp = c->T;
-        c->T = c->T2;
-        c->T2 = tmp;
+        std::swap(c->T, c->T2);
     }
     return BD_OK;
 }

@@ -140,6 +140,7 @@ _SIGS = {
     "bd_phase_slice": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                         ctypes.c_int64], ctypes.c_int),
     "bd_sync": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_get_tuning": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int], ctypes.c_int),
     "bd_epoch_q": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double,
                     ctypes.POINTER(EpochDiag)], ctypes.c_int),
     # surrogate coupling (bugdef_fno.h)
@@ -516,6 +517,12 @@ class Context:
 
     def sync(self):
         check(lib().bd_sync(self.handle), "bd_sync")
+
+    def tuning(self):
+        o = (ctypes.c_int32 * 6)()
+        check(lib().bd_get_tuning(self.handle, o, 6), "bd_get_tuning")
+        return {"substep_tile_y": o[0], "substep_tile_x": o[1], "pair_tile_y": o[2], "pair_tile_x": o[3],
+                "pairs": bool(o[4]), "tuned": bool(o[5])}
 
 
 _cache = {}

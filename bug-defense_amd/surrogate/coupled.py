@@ -7,8 +7,9 @@ last MC epoch outside the window) and the epoch runs without transport
 (``step_epoch_with_q``); every ``judge_every``-th epoch the plain MC epoch runs
 and judges the network, whose prediction for the same state is composed into a
 separate field and compared with the MC field.  When the judged relative L2
-error exceeds ``tol``, the loop falls back to MC on every epoch (still judging
-each one) until the error is back within ``tol``.
+error exceeds ``tol``, the loop falls back to MC on every epoch until a later
+judge epoch (still every ``judge_every``-th) finds the error back within
+``tol``; the fallback epochs in between are plain epochs, with no prediction.
 
 The judged error is the relative L2 over 4x4x4 block sums of the absorbed
 power (non-ablated voxels), which averages down the MC's own shot noise; the
@@ -55,16 +56,19 @@ class SurrogateCoupledEngine:
             d['optics_source'] = 'none'
             return d
         # MC on the first epoch (it also supplies the field outside the window), MC and a
-        # judgement on the next one, then every judge_every-th epoch and while falling back
-        judge = (not self.judged) or self.fallback or (eng.epoch % self.judge_every == 0)
+        # judgement on the next one, then a judgement every judge_every-th epoch; while
+        # falling back every epoch is MC, judged only on those epochs
+        judging = (not self.judged) or (eng.epoch % self.judge_every == 0)
+        judge = judging or self.fallback
         t0 = time.perf_counter()
         if judge:
             err = None
-            if self.have_stale:
+            compare = self.have_stale and judging
+            if compare:
                 beam = eng._epoch_beam()
                 self._predict(beam, into_q=False)
             d = eng.step_epoch()
-            if self.have_stale:
+            if compare:
                 err = eng._ctx.sur_error()
                 e = err[1] if self.metric == "block4" else err[0]
                 self.fallback = e > self.tol

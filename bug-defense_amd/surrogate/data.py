@@ -87,6 +87,52 @@ class Dataset:
         return (self.X.view(c, (nb, T, T, T, CIN)), self.t.view(c, (nb, T, T, T)), self.w.view(c, (nb, T, T, T)))
 
 
+def generate_coupled(n, n_photons=1_000_000, seed=0, shape=(192, 192, 192), epochs_per_run=150, snapshot_every=3,
+                     progress=None, device=0):
+    """Dataset of n windows taken from coupled runs of the layered-stack workload,
+    i.e. the states the coupled loop hands the network: every `snapshot_every`-th
+    epoch of a plain-MC run (random stack seed, power 20-80 W, angle 0-30 deg, spot
+    0.35-0.65 mm), the beam window of that state, its features (transport optics with
+    the plume), and as target a fresh n_photons MC over the whole domain."""
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.workloads import layered_stack
+    rng = np.random.default_rng(seed)
+    ds = Dataset(n, device=device)
+    i = 0
+    while i < n:
+        P = float(rng.uniform(20.0, 80.0))
+        th = float(rng.uniform(0.0, 30.0))
+        spot = float(rng.uniform(0.35, 0.65))
+        kw = layered_stack(shape=shape, seed=int(rng.integers(1 << 30)), power=P, theta_deg=th, spot_dia=spot,
+                           plume_grid=tuple(s // 2 for s in shape))
+        eng = CoupledAblationEngine(**kw)
+        try:
+            for ep in range(epochs_per_run):
+                eng.step_epoch()
+                if (ep + 1) % snapshot_every:
+                    continue
+                beam = eng._epoch_beam()
+                origin = eng._ctx.sur_window(beam)
+                eng._ctx.sur_features(beam, origin, ds.X.view(i, (T, T, T, CIN)))
+                beam.n_photons = int(n_photons)
+                beam.seed = int(rng.integers(1, 1 << 62))
+                beam.score_fluence = 1
+                eng._ctx.mc_run(beam)
+                eng._ctx.sur_targets(beam, origin, ds.t.view(i, (T, T, T)), ds.w.view(i, (T, T, T)))
+                ds.meta.append({'theta': math.radians(th), 'spot': spot, 'power': P, 'epoch': ep + 1,
+                                'front': origin[3]})
+                i += 1
+                if progress and i % progress == 0:
+                    eng._ctx.sync()
+                    print(f"surrogate coupled data: {i}/{n} windows", flush=True)
+                if i == n:
+                    break
+            eng._ctx.sync()
+        finally:
+            eng.close()
+    return ds
+
+
 def generate(n, n_photons=1_000_000, seed=0, progress=None, device=0, mc_seed_offset=0):
     """Dataset of n samples from the device MC (seeded, reproducible); the
     geometry depends on `seed` only, the MC streams also on mc_seed_offset."""

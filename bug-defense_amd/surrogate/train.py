@@ -78,6 +78,8 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from saved weights")
     ap.add_argument("--modes", default="6,6,8", help="kept modes kx,ky,kz")
     ap.add_argument("--eval-every", type=int, default=10)
+    ap.add_argument("--coupled", type=int, default=0, help="windows taken from coupled runs (added to --samples)")
+    ap.add_argument("--coupled-holdout", type=int, default=64)
     ap.add_argument("--np-len", type=int, default=80, help="transform length")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
@@ -85,9 +87,13 @@ def main(argv=None):
     ntr = (a.samples // nb) * nb
     nho = (a.holdout // nb) * nb
     t0 = time.perf_counter()
-    ds = data.generate(ntr, n_photons=a.photons, seed=a.seed, progress=1024)
+    nco = (a.coupled // nb) * nb
+    ncho = (a.coupled_holdout // nb) * nb if nco else 0
+    ds = data.generate(ntr, n_photons=a.photons, seed=a.seed, progress=1024) if ntr else None
     ho = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000)
     ho2 = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000, mc_seed_offset=1)
+    cds = data.generate_coupled(nco, n_photons=a.photons, seed=a.seed + 7, progress=512) if nco else None
+    cho = data.generate_coupled(ncho, n_photons=a.photons, seed=a.seed + 2000) if ncho else None
     t_data = time.perf_counter() - t0
     print(f"data: {ntr} train + {nho} held-out samples (+ independent MC of the held-out) in {t_data:.1f} s",
           flush=True)
@@ -98,27 +104,32 @@ def main(argv=None):
         net = FourierSurrogate(modes=modes, np_len=a.np_len, max_batch=nb, train=True)
         net.set_params(init_params(net.modes, seed=a.seed))
     rng = np.random.default_rng(a.seed + 1)
-    nchunk = ntr // nb
+    chunks = [(ds, c) for c in range(ntr // nb)] + [(cds, c) for c in range(nco // nb)]
+    nchunk = len(chunks)
     total = a.epochs * nchunk
     step = 0
     hist = []
     t1 = time.perf_counter()
     for ep in range(a.epochs):
         losses = []
-        for c in rng.permutation(nchunk):
+        for ci in rng.permutation(nchunk):
             lr = a.lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
-            X, t, w = ds.chunk(int(c), nb)
+            src, c = chunks[int(ci)]
+            X, t, w = src.chunk(int(c), nb)
             losses.append(net.train_step(X, t, w, lr, weight_decay=a.wd))
             step += 1
         rec = {"epoch": ep + 1, "loss": float(np.mean(losses)), "s": round(time.perf_counter() - t1, 1)}
         if (ep + 1) % a.eval_every == 0 and ep + 1 < a.epochs:
             rec.update(evaluate(net, ho, 0, nho, nb))
+            if cho:
+                rec.update({"coupled_" + k: v for k, v in evaluate(net, cho, 0, ncho, nb).items()})
         hist.append(rec)
         print(json.dumps(rec), flush=True)
     t_train = time.perf_counter() - t1
     summary = {"train_samples": ntr, "heldout_samples": nho, "photons": a.photons, "epochs": a.epochs,
                "modes": list(net.modes), "np_len": net.np_len, "data_s": round(t_data, 1),
-               "train_s": round(t_train, 1), "heldout": evaluate(net, ho, 0, nho, nb, other=ho2)}
+               "train_s": round(t_train, 1), "heldout": evaluate(net, ho, 0, nho, nb, other=ho2),
+               "coupled_windows": nco, "coupled_heldout": evaluate(net, cho, 0, ncho, nb) if cho else None}
     net.save(a.out, history=json.dumps(hist), summary=json.dumps(summary))
     print(json.dumps(summary), flush=True)
     net.close()

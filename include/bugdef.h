@@ -251,6 +251,12 @@ int bd_phase_slice(bd_ctx *ctx, int plane, int index, int stride, const uint8_t 
  * transmitted and lateral are zero and mc_energy_closure is NaN. */
 int bd_epoch_q(bd_ctx *ctx, const bd_beam *beam, int n_substeps, double t_after, bd_epoch_diag *out);
 
+/* Launch shapes the context's autotuner picked (no reference counterpart; for
+ * reports): out[0..5] = sub-step tile rows (y), tile planes (x), pair-kernel
+ * tile rows, pair-kernel tile planes, pairs of sub-steps fused in one launch
+ * (1/0), tuned yet (1/0).  n is the capacity of out (>= 6). */
+int bd_get_tuning(bd_ctx *ctx, int32_t *out, int n);
+
 /* Wait for all queued work of the context. */
 int bd_sync(bd_ctx *ctx);
 

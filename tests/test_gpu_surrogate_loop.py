@@ -68,9 +68,11 @@ def test_loop_judges_every_tenth_epoch_and_falls_back():
     eng.close()
     eng = _engine()
     sur = SurrogateCoupledEngine(eng, WEIGHTS, judge_every=5, tol=0.0)    # a tolerance everything exceeds
-    for _ in range(8):
+    for _ in range(11):
         sur.step_epoch()
-    assert [h["optics_source"] for h in sur.history] == ["mc"] * 8
+    assert [h["optics_source"] for h in sur.history] == ["mc"] * 11
     assert all(h["fallback"] for h in sur.history[1:])
+    # while falling back, only the judge epochs (1, 5, 10) compare against the network
+    assert [i for i, h in enumerate(sur.history) if h["judge_error"] is not None] == [1, 5, 10]
     sur.close()
     eng.close()

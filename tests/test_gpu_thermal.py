@@ -210,3 +210,38 @@ def test_voxel_thermal_solver_gpu_reference_fixture():
     assert res['engine'] == 'gpu'
     assert res['peak_surface_T'] >= 25.0 and not np.any(np.isnan(res['time_history']))
     assert len(res['time_history']) == 16
+
+
+@pytest.mark.parametrize("shape", [(37, 41, 125), (160, 136, 192)])
+def test_paired_substeps_bit_identical_to_single_substeps(shape, monkeypatch):
+    """The two-sub-step kernel (temporal blocking, ping-pong H / flags) against one launch
+    per sub-step: every field bit-identical after 22 sub-steps (10 pair launches + 2
+    single ones) with the plateau, Arrhenius, char, Q masking and T_peak, on partial
+    tiles in every axis and (second shape, above 1M voxels) the autotuned shapes."""
+    from simulation import _bdlib as B
+    rng = np.random.default_rng(11)
+    dx, dt = 0.015, 5e-5
+    k = rng.uniform(0.1e-3, 0.6e-3, shape)
+    rc = rng.uniform(2.0e-3, 4.2e-3, shape)
+    T0 = rng.uniform(60.0, 140.0, shape)
+    Q = rng.uniform(0.0, 400.0, shape)
+    dry = (rng.random(shape) < 0.2).astype(np.uint8)
+    abl = ((rng.random(shape) < 0.1) & (dry == 0)).astype(np.uint8)
+    H0 = np.where(rng.random(shape) < 0.3, rng.uniform(0, 0.3, shape), 0.0)
+    chi0 = rng.uniform(0.0, 0.5, shape)
+    out = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("BUGDEF_SUBSTEP_PAIRS", pairs)
+        c = _ctx(shape, (dx, dx, dx), dt=dt, h_eff=1.5e-5, t_ambient=20.0, t_abl=100.0, t_void=110.0,
+                 has_arrhenius=1, arr_A=3.1e98, arr_Ea_over_R=6.28e5 / 8.314, has_char=1, char_A=1.0e7,
+                 char_Ea_over_R=65000.0 / 8.314)
+        for name, a in (('T', T0), ('Q', Q), ('K', k), ('RHO_CP', rc), ('DRY', dry), ('ABLATED', abl),
+                        ('H_ACC', H0), ('CHI', chi0)):
+            c.upload(name, a)
+        c.fill('OMEGA', 0.0)
+        c.fill('T_PEAK', 0.0)
+        c.substeps(22, 1, B.SUB_PLATEAU | B.SUB_MASK_Q | B.SUB_ARRHENIUS | B.SUB_CHAR | B.SUB_PEAK)
+        out.append({n: c.download(n) for n in ('T', 'H_ACC', 'CHI', 'OMEGA', 'T_PEAK', 'ABLATED', 'DRY')})
+        c.close()
+    for n in out[0]:
+        np.testing.assert_array_equal(out[0][n], out[1][n], err_msg=n)
