@@ -3,19 +3,24 @@
 // tallies, composition of the absorbed-power field from a prediction, the
 // judge error against an MC field, and an epoch that consumes a supplied Q.
 //
-// Window: laterally centred on the beam axis, its top a few voxels above the
-// first non-void voxel the beam axis meets (the ablation front), so the beam
-// always enters the window through void as it does in the training samples.
+// Windows: the first is laterally centred on the beam axis, its top a few
+// voxels above the first non-void voxel the beam axis meets (the ablation
+// front), so the beam always enters it through void as it does in the training
+// samples.  Further windows are stacked above it along the beam axis, each
+// overlapping the one below by RAMP voxels, until one reaches RAMP voxels
+// above the tissue surface next to the crater (or the domain top): together
+// they cover the crater walls, where most of the power is absorbed once the
+// crater is deep.  Where windows overlap, their ramps sum to one.
 //
 // Features per voxel (channels last, [64][64][64][8]):
 //   0 log1p(mu_a)   1 log1p(mu_s / 10)   2 g   3 n - 1
 //   4 ballistic: ln(max(B, 1e-6)) / ln(1e6) + 1, B = collimated beam intensity
-//     relative to the beam's peak, marched plane by plane from the window top
+//     relative to the beam's peak, marched plane by plane from the domain top
 //     (planar Snell invariant n sin(theta), Fresnel transmission at index
 //     steps, Beer-Lambert attenuation mu_t dz / cos(theta), lateral drift
 //     dz tan(theta) by linear interpolation)
 //   5 sin(theta_in)   6 spot / 0.5 mm - 1
-//   7 optical depth along the beam from the window top, ln(1 + tau) / ln(101),
+//   7 optical depth along the beam from the domain top, ln(1 + tau) / ln(101),
 //     carried with the same drift as channel 4 (it keeps growing where the
 //     beam intensity has fallen below the channel-4 floor)
 // Optics are the packed transport optics the MC itself reads (plume included).
@@ -25,6 +30,7 @@
 // w = 0.5 Q^2 / sum Q^2 + 0.5 Phi / sum Phi (Q = mu_a Phi), summing to one.
 #include "bd_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace bd {
@@ -32,6 +38,9 @@ namespace bd {
 constexpr int SW = 64;                    // window edge
 constexpr float PHI_EPS = 1e-4f;          // offset of the log target
 constexpr int RAMP = 8;                   // blend width at window faces
+constexpr int MAXW = 4;                   // windows per beam (4 x 56 + 8 >= 232 planes)
+constexpr int FX = 256;                   // x extent of the feature march (window + 192 up-beam)
+constexpr int FUP = FX - SW;
 
 __device__ __forceinline__ float fresnel_u(float n1, float n2, float ci)
 {
@@ -63,100 +72,130 @@ __device__ __forceinline__ bool in_dom(const SurGeom &g, int X, int Y, int Z)
     return X >= 0 && X < g.Nx && Y >= 0 && Y < g.Ny && Z >= 0 && Z < g.Nz;
 }
 
-// window origin: first non-void voxel along the beam axis, minus the margin
-__global__ void k_sur_window(const float *__restrict__ mu_a, const float *__restrict__ mu_s,
-                             const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz, float dx, float dy, float dz,
-                             float xc, float yc, float tan0, int sdir, int margin, int *__restrict__ origin)
+// First non-void z along the beam axis (the ablation front) and first non-void
+// z over nine probe columns around the axis, +-16 and +-28 voxels in x and y
+// (the tissue surface next to the crater); block-parallel over z, min-reduced.
+__global__ __launch_bounds__(256) void k_sur_window(const float *__restrict__ mu_a, const float *__restrict__ mu_s,
+                                                    const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz,
+                                                    float dx, float dy, float dz, float xc, float yc, float tan0,
+                                                    int sdir, int *__restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ int sf[256], ss[256];
     const int Y = (int)floorf(yc / dy);
-    int front = Nz;
-    for (int z = 0; z < Nz; ++z) {
+    int front = Nz, surf = Nz;
+    for (int z = threadIdx.x; z < Nz; z += blockDim.x) {
         const float xm = xc + sdir * (z + 0.5f) * dz * tan0;
         const int X = (int)floorf(xm / dx);
-        if (X < 0 || X >= Nx || Y < 0 || Y >= Ny) continue;
-        const int64_t v = ((int64_t)X * Ny + Y) * Nz + z;
-        if ((flags[v] & F_ABL) == 0 && (mu_a[v] + mu_s[v]) > 0.f) {
-            front = z;
-            break;
+        for (int p = 0; p < 9; ++p) {
+            const int ox = p == 1 ? 16 : p == 2 ? -16 : p == 5 ? 28 : p == 6 ? -28 : 0;
+            const int oy = p == 3 ? 16 : p == 4 ? -16 : p == 7 ? 28 : p == 8 ? -28 : 0;
+            const int Xp = X + ox, Yp = Y + oy;
+            if (Xp < 0 || Xp >= Nx || Yp < 0 || Yp >= Ny) continue;
+            const int64_t v = ((int64_t)Xp * Ny + Yp) * Nz + z;
+            if ((flags[v] & F_ABL) == 0 && (mu_a[v] + mu_s[v]) > 0.f) {
+                if (p == 0) front = min(front, z);
+                surf = min(surf, z);
+            }
         }
     }
-    int z0 = front - margin;
-    z0 = z0 > Nz - SW ? Nz - SW : z0;
-    z0 = z0 < 0 ? 0 : z0;
-    const float xmid = (xc + sdir * (z0 + 0.5f * SW) * dz * tan0) / dx;
-    int x0 = (int)lrintf(xmid - 0.5f * SW);
-    x0 = x0 > Nx - SW ? Nx - SW : x0;
-    x0 = x0 < 0 ? 0 : x0;
-    int y0 = (int)lrintf(yc / dy - 0.5f * SW);
-    y0 = y0 > Ny - SW ? Ny - SW : y0;
-    y0 = y0 < 0 ? 0 : y0;
-    origin[0] = x0;
-    origin[1] = y0;
-    origin[2] = z0;
-    origin[3] = front;
+    sf[threadIdx.x] = front;
+    ss[threadIdx.x] = surf;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sf[threadIdx.x] = min(sf[threadIdx.x], sf[threadIdx.x + h]);
+            ss[threadIdx.x] = min(ss[threadIdx.x], ss[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sf[0];
+        out[1] = min(ss[0], sf[0]);
+    }
 }
 
 struct FeatArgs {
-    SurGeom g;
+    SurGeom g;                 // x0/y0/z0 unused: windows below
+    int4 win[MAXW];            // x0, y0, z0 of each window
     float sin0, cos0;
     int sdir;
-    float xa_top, yc, w0, n_amb;
+    float xc, yc, w0, n_amb;   // beam axis at the domain top
     float f5, f6;
-    float *x;
+    float *x;                  // [nwin][64][64][64][8]
 };
 
-// one block per window row j; threads cover x in [x0 - 64, x0 + 64) so light
-// drifting in from up-beam of the window is carried (drift < 1 voxel per plane)
-__global__ __launch_bounds__(128) void k_sur_features(FeatArgs a)
+// One block per (window row j, window).  The march runs from the domain top to
+// the window's last plane over x in [x0 - 192, x0 + 64) (mirrored for a beam
+// tilted towards -x), so light drifting into the window from up to 192 voxels
+// up-beam is carried; optics are loaded 8 planes ahead of the march, and the
+// neighbour exchange is double-buffered in LDS (one barrier per plane).
+__global__ __launch_bounds__(FX) void k_sur_features(FeatArgs a)
 {
     const SurGeom &g = a.g;
-    const int j = blockIdx.x, t = threadIdx.x;
-    const int X = g.x0 - SW + t, Y = g.y0 + j;
-    __shared__ float sB[128], sT[128];
+    const int j = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+    const int4 o = a.win[w];
+    const int xlo = a.sdir > 0 ? o.x - FUP : o.x;   // x of thread 0
+    const int tw = o.x - xlo;                       // first window thread
+    const int X = xlo + t, Y = o.y + j;
+    const bool col_in = X >= 0 && X < g.Nx && Y >= 0 && Y < g.Ny;
+    const int64_t colv = col_in ? ((int64_t)X * g.Ny + Y) * g.Nz : 0;
+    const int zend = o.z + SW;
+    __shared__ float sB[2][FX], sT[2][FX];
     const float xm = (X + 0.5f) * g.dx, ym = (Y + 0.5f) * g.dy;
-    const float u = (xm - a.xa_top) * a.cos0, v = ym - a.yc;
+    const float u = (xm - a.xc) * a.cos0, v = ym - a.yc;
     float B = expf(-2.f * (u * u + v * v) / (a.w0 * a.w0));
     float tau = 0.f;
     float nprev = a.n_amb;
     const float s0 = a.n_amb * a.sin0;
-    for (int k = 0; k < SW; ++k) {
-        const int Z = g.z0 + k;
-        float4 o = make_float4(0.f, 0.f, 0.f, a.n_amb);
-        if (in_dom(g, X, Y, Z)) o = g.opt[((int64_t)X * g.Ny + Y) * g.Nz + Z];
-        sB[t] = B;
-        sT[t] = tau;
-        __syncthreads();
-        const int tn = t - a.sdir;
-        const bool nb_in = tn >= 0 && tn < 128;
-        const float Bn = nb_in ? sB[tn] : B;
-        const float Tn = nb_in ? sT[tn] : tau;
-        __syncthreads();
-        const float sk = s0 / o.w;
-        float Bk = 0.f, tk = tau + (o.x + o.y) * g.dz;
-        if (sk < 1.f) {
-            const float ck = sqrtf(1.f - sk * sk);
-            const float sh = fminf(g.dz * sk / (ck * g.dx), 1.f);
-            const float bi = (1.f - sh) * B + sh * Bn;
-            const float sp = s0 / nprev;
-            const float cp = sqrtf(fmaxf(0.f, 1.f - sp * sp));
-            const float R = fresnel_u(nprev, o.w, cp);
-            Bk = bi * (1.f - R) * expf(-(o.x + o.y) * g.dz / ck);
-            tk = (1.f - sh) * tau + sh * Tn + (o.x + o.y) * g.dz / ck;
-        }
-        B = Bk;
-        tau = tk;
-        nprev = o.w;
-        if (t >= SW) {
-            float *xp = a.x + (((int64_t)(t - SW) * SW + j) * SW + k) * 8;
-            xp[0] = log1pf(o.x);
-            xp[1] = log1pf(o.y * 0.1f);
-            xp[2] = o.z;
-            xp[3] = o.w - 1.f;
-            xp[4] = logf(fmaxf(B, 1e-6f)) * (1.f / 13.815510557964274f) + 1.f;
-            xp[5] = a.f5;
-            xp[6] = a.f6;
-            xp[7] = log1pf(tau) * (1.f / 4.61512051684126f);
+    auto ld = [&](int Z) {
+        float4 r = make_float4(0.f, 0.f, 0.f, a.n_amb);
+        if (col_in && Z < g.Nz && Z < zend) r = g.opt[colv + Z];
+        return r;
+    };
+    float4 ring[8];
+#pragma unroll
+    for (int u8 = 0; u8 < 8; ++u8) ring[u8] = ld(u8);
+    float *xw = a.x + (int64_t)w * SW * SW * SW * 8;
+    int buf = 0;
+    for (int z8 = 0; z8 < zend; z8 += 8) {
+#pragma unroll
+        for (int u8 = 0; u8 < 8; ++u8) {
+            const int Z = z8 + u8;
+            if (Z >= zend) break;  // block-uniform
+            const float4 op = ring[u8];
+            ring[u8] = ld(Z + 8);
+            sB[buf][t] = B;
+            sT[buf][t] = tau;
+            __syncthreads();
+            const int tn = t - a.sdir;
+            const bool nb_in = tn >= 0 && tn < FX;
+            const float Bn = nb_in ? sB[buf][tn] : B;
+            const float Tn = nb_in ? sT[buf][tn] : tau;
+            buf ^= 1;
+            const float sk = s0 / op.w;
+            float Bk = 0.f, tk = tau + (op.x + op.y) * g.dz;
+            if (sk < 1.f) {
+                const float ck = sqrtf(1.f - sk * sk);
+                const float sh = fminf(g.dz * sk / (ck * g.dx), 1.f);
+                const float bi = (1.f - sh) * B + sh * Bn;
+                const float sp = s0 / nprev;
+                const float cp = sqrtf(fmaxf(0.f, 1.f - sp * sp));
+                const float R = fresnel_u(nprev, op.w, cp);
+                Bk = bi * (1.f - R) * expf(-(op.x + op.y) * g.dz / ck);
+                tk = (1.f - sh) * tau + sh * Tn + (op.x + op.y) * g.dz / ck;
+            }
+            B = Bk;
+            tau = tk;
+            nprev = op.w;
+            const int k = Z - o.z;
+            if (k >= 0 && t >= tw && t < tw + SW) {
+                float *xp = xw + (((int64_t)(t - tw) * SW + j) * SW + k) * 8;
+                const float4 f0 = make_float4(log1pf(op.x), log1pf(op.y * 0.1f), op.z, op.w - 1.f);
+                const float4 f1 = make_float4(logf(fmaxf(B, 1e-6f)) * (1.f / 13.815510557964274f) + 1.f, a.f5, a.f6,
+                                              log1pf(tau) * (1.f / 4.61512051684126f));
+                reinterpret_cast<float4 *>(xp)[0] = f0;
+                reinterpret_cast<float4 *>(xp)[1] = f1;
+            }
         }
     }
 }
@@ -229,21 +268,44 @@ __device__ __forceinline__ float ramp(int i, bool lo, bool hi)
     return r;
 }
 
-// Q over the window from the prediction, blended into the stale field at the
-// window faces that lie inside the domain
-__global__ void k_sur_apply(SurGeom g, const float *__restrict__ y, const float *__restrict__ Qs,
-                            float *__restrict__ Qo, float E0, int blend_mask)
+struct ApplyArgs {
+    SurGeom g;              // x0/y0/z0: corner of the windows' bounding box
+    int4 win[MAXW];         // x0, y0, z0, blend mask of each window
+    int nwin;
+    int bx, by, bz;         // bounding-box extent
+    const float *y;         // [nwin][64^3] predicted log fluence
+    const float *Qs;
+    float *Qo;
+    float E0;
+};
+
+// Q over the union of the windows: per voxel, s = sum_k r_k q_k and W = sum_k
+// r_k over the windows holding it (r_k: product of the face ramps of window k
+// at its faces inside the domain); Q = s / max(W, 1) + (1 - min(W, 1)) Q_stale.
+// In an overlap of RAMP voxels the two ramps sum to one, so the windows hand
+// over to each other; at the outer faces the prediction blends into the stale
+// field.
+__global__ void k_sur_apply(ApplyArgs a)
 {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)SW * SW * SW;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int k = (int)(t % SW), j = (int)((t / SW) % SW), i = (int)(t / (SW * SW));
-        const int X = g.x0 + i, Y = g.y0 + j, Z = g.z0 + k;
+    const SurGeom &g = a.g;
+    const int64_t nb = (int64_t)a.bx * a.by * a.bz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nb; t += (int64_t)gridDim.x * blockDim.x) {
+        const int Z = g.z0 + (int)(t % a.bz), Y = g.y0 + (int)((t / a.bz) % a.by), X = g.x0 + (int)(t / ((int64_t)a.bz * a.by));
         if (!in_dom(g, X, Y, Z)) continue;
         const int64_t v = ((int64_t)X * g.Ny + Y) * g.Nz + Z;
-        const float qs = E0 * g.opt[v].x * fmaxf(expf(y[t]) - PHI_EPS, 0.f);
-        const float r = ramp(i, blend_mask & 1, blend_mask & 2) * ramp(j, blend_mask & 4, blend_mask & 8) *
-                        ramp(k, blend_mask & 16, blend_mask & 32);
-        Qo[v] = r * qs + (1.f - r) * Qs[v];
+        float sq = 0.f, W = 0.f;
+        const float ma = g.opt[v].x;
+        for (int k = 0; k < a.nwin; ++k) {
+            const int4 o = a.win[k];
+            const int i = X - o.x, j = Y - o.y, l = Z - o.z;
+            if (i < 0 || i >= SW || j < 0 || j >= SW || l < 0 || l >= SW) continue;
+            const int m = o.w;
+            const float r = ramp(i, m & 1, m & 2) * ramp(j, m & 4, m & 8) * ramp(l, m & 16, m & 32);
+            const float yv = a.y[(int64_t)k * SW * SW * SW + ((int64_t)i * SW + j) * SW + l];
+            sq += r * (a.E0 * ma * fmaxf(expf(yv) - PHI_EPS, 0.f));
+            W += r;
+        }
+        if (W > 0.f) a.Qo[v] = sq / fmaxf(W, 1.f) + (1.f - fminf(W, 1.f)) * a.Qs[v];
     }
 }
 
@@ -368,51 +430,87 @@ using namespace bd;
 
 extern "C" {
 
-int bd_sur_window(bd_ctx *h, const bd_beam *beam, int margin, int32_t *origin)
+int bd_sur_windows(bd_ctx *h, const bd_beam *beam, int margin, int max_windows, int32_t *origins, int *n_windows)
 {
-    BD_CHECK(h && beam && origin, "bd_sur_window: null argument");
-    BD_CHECK(margin >= 0 && margin < SW, "bd_sur_window: margin must be in [0, 64)");
+    BD_CHECK(h && beam && origins && n_windows, "bd_sur_windows: null argument");
+    BD_CHECK(margin >= 0 && margin < SW, "bd_sur_windows: margin must be in [0, 64)");
+    BD_CHECK(max_windows >= 1 && max_windows <= MAXW, "bd_sur_windows: max_windows must be in [1, 4]");
     Ctx *c = &h->c;
     BD_TRY(ensure_state(c));
-    BD_CHECK(c->mu_a && c->mu_s, "bd_sur_window: optics fields are not uploaded");
+    BD_CHECK(c->mu_a && c->mu_s, "bd_sur_windows: optics fields are not uploaded");
     BeamFrame f;
     BD_TRY(beam_frame(beam, f));
     int *dev = (int *)(c->counter + 8);
-    hipLaunchKernelGGL(k_sur_window, dim3(1), dim3(64), 0, c->stream, c->mu_a, c->mu_s, c->flags, c->g.nx, c->g.ny,
+    hipLaunchKernelGGL(k_sur_window, dim3(1), dim3(256), 0, c->stream, c->mu_a, c->mu_s, c->flags, c->g.nx, c->g.ny,
                        c->g.nz, (float)c->g.dx, (float)c->g.dy, (float)c->g.dz, (float)beam->x_center,
-                       (float)beam->y_center, f.tan0, f.sdir, margin, dev);
+                       (float)beam->y_center, f.tan0, f.sdir, dev);
     BD_HIP(hipGetLastError());
-    BD_HIP(hipMemcpyAsync(c->host_pinned, dev, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipMemcpyAsync(c->host_pinned, dev, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
-    const int *r = (const int *)c->host_pinned;
-    for (int d = 0; d < 4; ++d) origin[d] = r[d];
+    const int front = ((const int *)c->host_pinned)[0], surf = ((const int *)c->host_pinned)[1];
+    const int Nx = c->g.nx, Ny = c->g.ny, Nz = c->g.nz;
+    const double tan0 = std::tan(beam->theta_rad);
+    int y0 = (int)std::nearbyint(beam->y_center / c->g.dy - 0.5 * SW);
+    y0 = std::max(0, std::min(y0, Ny - SW));
+    int z0 = std::max(0, std::min(front - margin, Nz - SW));
+    int n = 0;
+    for (;;) {
+        const double xmid = (beam->x_center + f.sdir * (z0 + 0.5 * SW) * c->g.dz * tan0) / c->g.dx;
+        int x0 = (int)std::nearbyint(xmid - 0.5 * SW);
+        x0 = std::max(0, std::min(x0, Nx - SW));
+        origins[4 * n] = x0;
+        origins[4 * n + 1] = y0;
+        origins[4 * n + 2] = z0;
+        origins[4 * n + 3] = front;
+        ++n;
+        if (z0 <= 0 || z0 <= surf - RAMP || n == max_windows) break;
+        z0 = std::max(z0 - (SW - RAMP), 0);
+    }
+    *n_windows = n;
     return BD_OK;
 }
 
-int bd_sur_features(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float *x_dev)
+int bd_sur_window(bd_ctx *h, const bd_beam *beam, int margin, int32_t *origin)
 {
-    BD_CHECK(h && beam && x_dev, "bd_sur_features: null argument");
+    BD_CHECK(origin, "bd_sur_window: null argument");
+    int32_t o[4 * MAXW];
+    int n = 0;
+    BD_TRY(bd_sur_windows(h, beam, margin, 1, o, &n));
+    for (int d = 0; d < 4; ++d) origin[d] = o[d];
+    return BD_OK;
+}
+
+int bd_sur_features_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins, float *x_dev)
+{
+    BD_CHECK(h && beam && origins && x_dev, "bd_sur_features: null argument");
+    BD_CHECK(n >= 1 && n <= MAXW, "bd_sur_features: window count must be in [1, 4]");
     Ctx *c = &h->c;
     BD_TRY(ensure_state(c));
-    BD_TRY(check_origin(c, origin));
+    for (int k = 0; k < n; ++k) BD_TRY(check_origin(c, origins + 4 * k));
     BeamFrame f;
     BD_TRY(beam_frame(beam, f));
     BD_TRY(launch_pack_optics(c, beam->add_plume != 0));
     FeatArgs a{};
-    a.g = geom(c, origin);
+    a.g = geom(c, origins);
+    for (int k = 0; k < n; ++k) a.win[k] = make_int4(origins[4 * k], origins[4 * k + 1], origins[4 * k + 2], 0);
     a.sin0 = f.sin0;
     a.cos0 = f.cos0;
     a.sdir = f.sdir;
-    a.xa_top = (float)(beam->x_center + f.sdir * origin[2] * c->g.dz * f.tan0);
+    a.xc = (float)beam->x_center;
     a.yc = (float)beam->y_center;
     a.w0 = f.w0;
     a.n_amb = (float)beam->n_ambient;
     a.f5 = f.sin0;
     a.f6 = (float)(beam->spot_dia / 0.5 - 1.0);
     a.x = x_dev;
-    hipLaunchKernelGGL(k_sur_features, dim3(SW), dim3(128), 0, c->stream, a);
+    hipLaunchKernelGGL(k_sur_features, dim3(SW, n), dim3(FX), 0, c->stream, a);
     BD_HIP(hipGetLastError());
     return BD_OK;
+}
+
+int bd_sur_features(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float *x_dev)
+{
+    return bd_sur_features_n(h, beam, 1, origin, x_dev);
 }
 
 int bd_sur_targets(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float *t_dev, float *w_dev)
@@ -435,12 +533,13 @@ int bd_sur_targets(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float 
     return BD_OK;
 }
 
-int bd_sur_apply(bd_ctx *h, const bd_beam *beam, const int32_t *origin, const float *y_dev, int into_q)
+int bd_sur_apply_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins, const float *y_dev, int into_q)
 {
-    BD_CHECK(h && beam && y_dev, "bd_sur_apply: null argument");
+    BD_CHECK(h && beam && origins && y_dev, "bd_sur_apply: null argument");
+    BD_CHECK(n >= 1 && n <= MAXW, "bd_sur_apply: window count must be in [1, 4]");
     Ctx *c = &h->c;
     BD_TRY(ensure_state(c));
-    BD_TRY(check_origin(c, origin));
+    for (int k = 0; k < n; ++k) BD_TRY(check_origin(c, origins + 4 * k));
     BD_CHECK(c->optics, "bd_sur_apply: optics are not packed (call bd_sur_features first)");
     BeamFrame f;
     BD_TRY(beam_frame(beam, f));
@@ -453,17 +552,38 @@ int bd_sur_apply(bd_ctx *h, const bd_beam *beam, const int32_t *origin, const fl
     const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
     hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(256), 0, c->stream, c->Qs, dst, c->nvox);
     BD_HIP(hipGetLastError());
-    const float E0 = (float)(beam->power / (M_PI * (double)f.w0 * f.w0 / 2.0));
-    int mask = 0;
-    const int n[3] = {c->g.nx, c->g.ny, c->g.nz};
-    for (int d = 0; d < 3; ++d) {
-        if (origin[d] > 0) mask |= 1 << (2 * d);
-        if (origin[d] + SW < n[d]) mask |= 2 << (2 * d);
+    ApplyArgs a{};
+    const int dims[3] = {c->g.nx, c->g.ny, c->g.nz};
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {0, 0, 0};
+    for (int k = 0; k < n; ++k) {
+        const int32_t *o = origins + 4 * k;
+        int mask = 0;
+        for (int d = 0; d < 3; ++d) {
+            if (o[d] > 0) mask |= 1 << (2 * d);
+            if (o[d] + SW < dims[d]) mask |= 2 << (2 * d);
+            lo[d] = std::min(lo[d], o[d]);
+            hi[d] = std::max(hi[d], std::min(o[d] + SW, dims[d]));
+        }
+        a.win[k] = make_int4(o[0], o[1], o[2], mask);
     }
-    hipLaunchKernelGGL(k_sur_apply, dim3(1024), dim3(256), 0, c->stream, geom(c, origin), y_dev, c->Qs, dst, E0,
-                       mask);
+    const int32_t corner[3] = {lo[0], lo[1], lo[2]};
+    a.g = geom(c, corner);
+    a.nwin = n;
+    a.bx = hi[0] - lo[0];
+    a.by = hi[1] - lo[1];
+    a.bz = hi[2] - lo[2];
+    a.y = y_dev;
+    a.Qs = c->Qs;
+    a.Qo = dst;
+    a.E0 = (float)(beam->power / (M_PI * (double)f.w0 * f.w0 / 2.0));
+    hipLaunchKernelGGL(k_sur_apply, dim3(2048), dim3(256), 0, c->stream, a);
     BD_HIP(hipGetLastError());
     return BD_OK;
+}
+
+int bd_sur_apply(bd_ctx *h, const bd_beam *beam, const int32_t *origin, const float *y_dev, int into_q)
+{
+    return bd_sur_apply_n(h, beam, 1, origin, y_dev, into_q);
 }
 
 int bd_sur_keep_stale(bd_ctx *h)

@@ -37,7 +37,7 @@ struct SubArgs {
     int ntz, nty, ntx, xc;
     float idx2, idy2, idz2, dz, inv_h, h_eff;
     float dt, t_amb, t_abl, t_void, qscale;
-    float arrA, arrEaR, chA, chEaR;
+    float arrA, arrEaR, chA, chEaR, ch_tcold;
     int plateau, arr, chr, maskq, peak;
     double vol_dt;
     // two-sub-step kernel: H and flags are read from one buffer and written to another
@@ -61,7 +61,7 @@ struct StepK {
     int Nx, Ny, Nz;
     float idx2, idy2, idz2, dz, inv_h, h_eff;
     float dt, t_amb, t_abl, t_void, qscale;
-    float arrA, arrEaR, chA, chEaR;
+    float arrA, arrEaR, chA, chEaR, ch_tcold;
     int plateau, arr, chr, maskq;
 };
 
@@ -71,7 +71,7 @@ __device__ __forceinline__ StepK step_consts(const SubArgs &a)
     k.Nx = a.Nx; k.Ny = a.Ny; k.Nz = a.Nz;
     k.idx2 = a.idx2; k.idy2 = a.idy2; k.idz2 = a.idz2; k.dz = a.dz; k.inv_h = a.inv_h; k.h_eff = a.h_eff;
     k.dt = a.dt; k.t_amb = a.t_amb; k.t_abl = a.t_abl; k.t_void = a.t_void; k.qscale = a.qscale;
-    k.arrA = a.arrA; k.arrEaR = a.arrEaR; k.chA = a.chA; k.chEaR = a.chEaR;
+    k.arrA = a.arrA; k.arrEaR = a.arrEaR; k.chA = a.chA; k.chEaR = a.chEaR; k.ch_tcold = a.ch_tcold;
     k.plateau = a.plateau; k.arr = a.arr; k.chr = a.chr; k.maskq = a.maskq;
     return k;
 }
@@ -131,15 +131,68 @@ __device__ __forceinline__ float cell_step(const StepK &a, int x, int y, int z, 
     return Tn;
 }
 
+// Char kinetics (coupled_ablation.py:48-58) run on dry, intact cells above
+// ch_tcold.  Below it rate dt < 2^-26, where e^{-rate dt} rounds to 1 in FP32
+// and the FP32 update 1 - (1 - chi) e^{-rate dt} leaves chi as it is (up to the
+// rounding of 1 - (1 - chi)); such a cell keeps its chi, which is then neither
+// read nor written.
+__device__ __forceinline__ bool char_active(const StepK &a, float Tn, uint8_t fl)
+{
+    return a.chr && (fl & F_DRY) && !(fl & F_ABL) && Tn > a.ch_tcold;
+}
+
+__device__ __forceinline__ float char_step(const StepK &a, float Tn, float chi)
+{
+#pragma clang fp contract(off)
+    const float rate = a.chA * expf(-a.chEaR / (Tn + 273.15f));
+    return 1.0f - (1.0f - chi) * expf(-rate * a.dt);
+}
+
+// Arrhenius damage integral (coupled_ablation.py:33-45), on intact cells
+__device__ __forceinline__ float arr_step(const StepK &a, float Tn, float om)
+{
+#pragma clang fp contract(off)
+    return om + a.dt * a.arrA * expf(-a.arrEaR / (Tn + 273.15f));
+}
+
 // Arrhenius damage and char kinetics of one cell at its new temperature
 // (coupled_ablation.py:33-58)
 __device__ __forceinline__ void cell_damage(const StepK &a, float Tn, uint8_t fl, float &om, float &chi)
 {
-#pragma clang fp contract(off)
-    if (a.arr && !(fl & F_ABL)) om += a.dt * a.arrA * expf(-a.arrEaR / (Tn + 273.15f));
-    if (a.chr && (fl & F_DRY) && !(fl & F_ABL)) {
-        const float rate = a.chA * expf(-a.chEaR / (Tn + 273.15f));
-        chi = 1.0f - (1.0f - chi) * expf(-rate * a.dt);
+    if (a.arr && !(fl & F_ABL)) om = arr_step(a, Tn, om);
+    if (char_active(a, Tn, fl)) chi = char_step(a, Tn, chi);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Adds a block's conducted boundary energy to the ledger.  Only the cells at
+// z == 0 and z == Nz-1 carry a boundary flux, so only the lanes holding those
+// two planes can be non-zero: each wave reads them out of its lanes (no LDS
+// tree, no barrier in blocks away from the z faces), the block sums its waves
+// in wave order and issues at most one atomic.  zlane0 is the z of lane 0;
+// lanes below `lmin` are halo lanes whose econd is zero anyway.
+template <int BY>
+__device__ __forceinline__ void ledger_flush(const SubArgs &a, double econd, int zlane0, int lmin, int wy, int lane)
+{
+    const int l_top = -zlane0, l_bot = a.Nz - 1 - zlane0;
+    const bool has_top = l_top >= lmin && l_top < 64, has_bot = l_bot >= lmin && l_bot < 64 && l_bot != l_top;
+    if (!(has_top || has_bot)) return;  // block-uniform
+    double w = 0.0;
+    if (has_top) w += readlane_f64(econd, l_top);
+    if (has_bot) w += readlane_f64(econd, l_bot);
+    __shared__ double red[BY];
+    if (lane == 0) red[wy] = w;
+    __syncthreads();
+    if (wy == 0 && lane == 0) {
+        double s = 0.0;
+        for (int i = 0; i < BY; ++i) s += red[i];
+        if (s != 0.0) unsafeAtomicAdd(a.ledger, s * a.vol_dt);
     }
 }
 
@@ -189,7 +242,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
     int buf = 0;
     // per-cell inputs of the next plane, loaded one iteration ahead so their
     // latency overlaps the current plane's stencil
-    float cap_n = 1.f, q_n = 0.f, h_n = 0.f, chi_n = 0.f;
+    float cap_n = 1.f, q_n = 0.f, h_n = 0.f;
     uint8_t fl_n = 0;
     if (valid && x0 < x1) {
         const int64_t id0 = x0 * plane + col;
@@ -197,7 +250,6 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
         q_n = a.Q[id0];
         fl_n = a.flags[id0];
         if (a.plateau && (fl_n & F_HPOS)) h_n = a.H[id0];
-        if (a.chr && (fl_n & F_DRY)) chi_n = a.chi[id0];
     }
     // halo cells of a plane (y rows by the edge waves, z ends by the edge lanes),
     // also loaded one plane ahead
@@ -247,7 +299,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
             }
         }
         float Tn2 = 0.f, kn2 = 0.f;
-        const float cap = cap_n, qv = q_n, h0 = h_n, chi0 = chi_n;
+        const float cap = cap_n, qv = q_n, h0 = h_n;
         const uint8_t fl = fl_n;
         const int64_t id = pbase + col;
         if (valid) {
@@ -261,7 +313,6 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
                 q_n = a.Q[idn];
                 fl_n = a.flags[idn];
                 h_n = (a.plateau && (fl_n & F_HPOS)) ? a.H[idn] : 0.f;
-                chi_n = (a.chr && (fl_n & F_DRY)) ? a.chi[idn] : 0.f;
             }
         }
         __syncthreads();
@@ -274,12 +325,9 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
                                        sK[buf][wy + 1][lane + 2], cap, qv, fl, h, fl2, &fbound);
             econd += (double)fbound;
             if (fl2 != fl || h != h0) a.H[id] = h;
-            if (a.arr || a.chr) {
-                float om = a.arr ? a.Om[id] : 0.f, chi = chi0;
-                cell_damage(K, Tn, fl, om, chi);
-                if (a.arr && !(fl & F_ABL)) a.Om[id] = om;
-                if (a.chr && (fl & F_DRY) && !(fl & F_ABL)) a.chi[id] = chi;
-            }
+            if (a.arr && !(fl & F_ABL)) a.Om[id] = arr_step(K, Tn, a.Om[id]);
+            // chi is read only where the char step runs (hot, dry, intact cells)
+            if (char_active(K, Tn, fl)) a.chi[id] = char_step(K, Tn, a.chi[id]);
             a.Tout[id] = Tn;
             if (a.peak) a.Tpeak[id] = fmaxf(a.Tpeak[id], Tn);
             if (fl2 != fl) a.flags[id] = fl2;
@@ -289,18 +337,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
         Tp = Tn2; kp = kn2;
         buf ^= 1;
     }
-    if (a.ledger) {
-        // block reduction of the boundary energy (only boundary cells contribute)
-        __shared__ double red[64 * BY];
-        red[wy * 64 + lane] = econd;
-        __syncthreads();
-        for (int w = 32 * BY; w > 0; w >>= 1) {
-            const int tid = wy * 64 + lane;
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
-        }
-        if (wy == 0 && lane == 0 && red[0] != 0.0) unsafeAtomicAdd(a.ledger, red[0] * a.vol_dt);
-    }
+    if (a.ledger) ledger_flush<BY>(a, econd, z - lane, 0, wy, lane);
 }
 
 // Two sub-steps per launch (temporal blocking).  A block owns a 60(z) x
@@ -421,7 +458,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep2(SubArgs a)
             if ((fl2 | fl_prev_out) & F_HPOS) a.H[id] = (fl2 & F_HPOS) ? h2 : 0.f;
             a.flags[id] = fl2;
             if (a.arr && !(fl1q & F_ABL)) a.Om[id] = om2;
-            if (a.chr && (fl1q & F_DRY) && !(fl1q & F_ABL)) a.chi[id] = chi2;
+            if (char_active(K, T1c, cq.fl) || char_active(K, T2, fl1q)) a.chi[id] = chi2;
         }
         // roll the plane registers
         T0m = T0c;
@@ -443,17 +480,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep2(SubArgs a)
         kbp = kb;
         kb = kb == 2 ? 0 : kb + 1;
     }
-    if (a.ledger) {
-        __shared__ double red[64 * BY];
-        red[wy * 64 + lane] = econd;
-        __syncthreads();
-        for (int w = 32 * BY; w > 0; w >>= 1) {
-            const int tid = wy * 64 + lane;
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
-        }
-        if (wy == 0 && lane == 0 && red[0] != 0.0) unsafeAtomicAdd(a.ledger, red[0] * a.vol_dt);
-    }
+    if (a.ledger) ledger_flush<BY>(a, econd, z - lane, 2, wy, lane);
 }
 
 static int launch_one(Ctx *c, SubArgs &a, int by)
@@ -522,6 +549,12 @@ static void fill_args(Ctx *c, SubArgs &a, int is_active, int flags)
     a.arrEaR = (float)p.arr_Ea_over_R;
     a.chA = (float)p.char_A;
     a.chEaR = (float)p.char_Ea_over_R;
+    // temperature below which one step's rate dt stays under 2^-26:
+    // A dt e^{-Ea/(R Tk)} < 2^-26  <=>  Tk < (Ea/R) / ln(A dt 2^26), with 0.1% margin
+    {
+        const double x = p.char_A * p.dt_sub * std::ldexp(1.0, 26);
+        a.ch_tcold = x > 1.0 ? (float)(0.999 * p.char_Ea_over_R / std::log(x) - 273.15) : 3.0e38f;
+    }
     a.plateau = (flags & BD_SUB_PLATEAU) ? 1 : 0;
     a.arr = ((flags & BD_SUB_ARRHENIUS) && p.has_arrhenius) ? 1 : 0;
     a.chr = ((flags & BD_SUB_CHAR) && p.has_char) ? 1 : 0;

@@ -152,6 +152,12 @@ _SIGS = {
                         ctypes.c_void_p], ctypes.c_int),
     "bd_sur_apply": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p,
                       ctypes.c_int], ctypes.c_int),
+    "bd_sur_windows": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_int,
+                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "bd_sur_features_n": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                           ctypes.c_void_p], ctypes.c_int),
+    "bd_sur_apply_n": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                        ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "bd_sur_keep_stale": ([ctypes.c_void_p], ctypes.c_int),
     "bd_sur_error": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     # surrogate network (bugdef_fno.h); tensors are device pointers
@@ -482,6 +488,34 @@ class Context:
     def sur_apply(self, beam: Beam, origin, y_dev, into_q):
         check(lib().bd_sur_apply(self.handle, ctypes.byref(beam), self._origin(origin), y_dev.ptr,
                                  1 if into_q else 0), "bd_sur_apply")
+
+    def sur_windows(self, beam: Beam, margin=12, max_windows=4):
+        """Tiled windows, bottom first: a list of (x0, y0, z0, front)."""
+        o = (ctypes.c_int32 * 16)()
+        n = ctypes.c_int(0)
+        check(lib().bd_sur_windows(self.handle, ctypes.byref(beam), int(margin), int(max_windows), o,
+                                   ctypes.byref(n)), "bd_sur_windows")
+        return [tuple(int(o[4 * k + d]) for d in range(4)) for k in range(n.value)]
+
+    @staticmethod
+    def _origins(origins):
+        flat = [int(v) for o in origins for v in (tuple(o[:3]) + (0,))]
+        return (ctypes.c_int32 * len(flat))(*flat), len(origins)
+
+    def sur_features_n(self, beam: Beam, origins, x_dev):
+        """Features of len(origins) windows into x_dev [n, 64, 64, 64, 8]."""
+        o, n = self._origins(origins)
+        if x_dev.shape[0] < n:
+            raise ValueError("sur_features_n: x_dev holds fewer windows than origins")
+        check(lib().bd_sur_features_n(self.handle, ctypes.byref(beam), n, o, x_dev.ptr), "bd_sur_features_n")
+
+    def sur_apply_n(self, beam: Beam, origins, y_dev, into_q):
+        """Compose Q from predictions y_dev [n, 64, 64, 64] of len(origins) windows."""
+        o, n = self._origins(origins)
+        if y_dev.shape[0] < n:
+            raise ValueError("sur_apply_n: y_dev holds fewer windows than origins")
+        check(lib().bd_sur_apply_n(self.handle, ctypes.byref(beam), n, o, y_dev.ptr, 1 if into_q else 0),
+              "bd_sur_apply_n")
 
     def sur_keep_stale(self):
         check(lib().bd_sur_keep_stale(self.handle), "bd_sur_keep_stale")

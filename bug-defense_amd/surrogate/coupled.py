@@ -1,9 +1,11 @@
 """The coupled epoch loop with the surrogate in it (judge / fallback).
 
 ``SurrogateCoupledEngine`` drives a ``CoupledAblationEngine``: on most epochs
-the network supplies the absorbed power over the 64^3 beam window (features
-from the current optics, one forward pass, Q composed with the field of the
-last MC epoch outside the window) and the epoch runs without transport
+the network supplies the absorbed power over the tiled beam windows (up to
+four 64^3 windows stacked along the beam from below the ablation front to
+above the tissue surface, overlapping by 8 voxels: features from the current
+optics, one batched forward pass, Q composed over the windows and with the
+field of the last MC epoch outside them) and the epoch runs without transport
 (``step_epoch_with_q``); every ``judge_every``-th epoch the plain MC epoch runs
 and judges the network, whose prediction for the same state is composed into a
 separate field and compared with the MC field.  When the judged relative L2
@@ -24,17 +26,22 @@ from .network import CIN, T, FourierSurrogate
 
 
 class SurrogateCoupledEngine:
-    def __init__(self, engine, weights, judge_every=10, tol=0.05, margin=12, metric="block4"):
+    def __init__(self, engine, weights, judge_every=10, tol=0.05, margin=12, metric="block4", max_windows=4):
         if engine.plume is not None and engine.plume.model != 'diffusion':
             raise ValueError("the surrogate loop supports the diffusion plume only")
         self.engine = engine
-        self.net = weights if isinstance(weights, FourierSurrogate) else FourierSurrogate.load(weights, max_batch=1)
+        self.max_windows = int(max_windows)
+        self.net = weights if isinstance(weights, FourierSurrogate) else FourierSurrogate.load(
+            weights, max_batch=self.max_windows)
+        if self.net.max_batch < self.max_windows:
+            raise ValueError("the network's max_batch is smaller than max_windows")
         self.judge_every = int(judge_every)
         self.tol = float(tol)
         self.margin = int(margin)
         self.metric = metric
-        self.x = DeviceArray((1, T, T, T, CIN))
-        self.y = DeviceArray((1, T, T, T))
+        self.x = DeviceArray((self.max_windows, T, T, T, CIN))
+        self.y = DeviceArray((self.max_windows, T, T, T))
+        self.windows = []
         self.fallback = False
         self.have_stale = False
         self.judged = False
@@ -42,12 +49,14 @@ class SurrogateCoupledEngine:
 
     def _predict(self, beam, into_q):
         ctx = self.engine._ctx
-        origin = ctx.sur_window(beam, self.margin)
-        ctx.sur_features(beam, origin, self.x.view(0, (T, T, T, CIN)))
+        ws = ctx.sur_windows(beam, self.margin, self.max_windows)
+        nb = len(ws)
+        ctx.sur_features_n(beam, ws, self.x)
         ctx.sync()
-        self.net.forward(self.x, self.y)
-        ctx.sur_apply(beam, origin, self.y.view(0, (T, T, T)), into_q)
-        return origin
+        self.net.forward(self.x.view(0, (nb, T, T, T, CIN)), self.y.view(0, (nb, T, T, T)))
+        ctx.sur_apply_n(beam, ws, self.y, into_q)
+        self.windows = ws
+        return ws
 
     def step_epoch(self) -> dict:
         eng = self.engine
@@ -58,6 +67,7 @@ class SurrogateCoupledEngine:
         # MC on the first epoch (it also supplies the field outside the window), MC and a
         # judgement on the next one, then a judgement every judge_every-th epoch; while
         # falling back every epoch is MC, judged only on those epochs
+        self.windows = []
         judging = (not self.judged) or (eng.epoch % self.judge_every == 0)
         judge = judging or self.fallback
         t0 = time.perf_counter()
@@ -85,7 +95,8 @@ class SurrogateCoupledEngine:
             d['judge_error'] = None
         d['t_wall_s'] = time.perf_counter() - t0
         d['fallback'] = self.fallback
-        self.history.append({k: d[k] for k in ('epoch', 'optics_source', 'judge_error', 'fallback', 'n_new')})
+        d['windows'] = len(self.windows)
+        self.history.append({k: d[k] for k in ('epoch', 'optics_source', 'judge_error', 'fallback', 'n_new', 'windows')})
         return d
 
     def close(self):

@@ -88,12 +88,13 @@ class Dataset:
 
 
 def generate_coupled(n, n_photons=1_000_000, seed=0, shape=(192, 192, 192), epochs_per_run=150, snapshot_every=3,
-                     progress=None, device=0):
+                     progress=None, device=0, max_windows=4):
     """Dataset of n windows taken from coupled runs of the layered-stack workload,
     i.e. the states the coupled loop hands the network: every `snapshot_every`-th
     epoch of a plain-MC run (random stack seed, power 20-80 W, angle 0-30 deg, spot
-    0.35-0.65 mm), the beam window of that state, its features (transport optics with
-    the plume), and as target a fresh n_photons MC over the whole domain."""
+    0.35-0.65 mm), the tiled beam windows of that state (up to max_windows), their
+    features (transport optics with the plume), and as targets one fresh n_photons
+    MC over the whole domain."""
     from simulation.coupled_ablation import CoupledAblationEngine
     from simulation.workloads import layered_stack
     rng = np.random.default_rng(seed)
@@ -112,17 +113,20 @@ def generate_coupled(n, n_photons=1_000_000, seed=0, shape=(192, 192, 192), epoc
                 if (ep + 1) % snapshot_every:
                     continue
                 beam = eng._epoch_beam()
-                origin = eng._ctx.sur_window(beam)
-                eng._ctx.sur_features(beam, origin, ds.X.view(i, (T, T, T, CIN)))
+                ws = eng._ctx.sur_windows(beam, 12, max_windows)[:n - i]
+                for k, o in enumerate(ws):
+                    eng._ctx.sur_features(beam, o, ds.X.view(i + k, (T, T, T, CIN)))
                 beam.n_photons = int(n_photons)
                 beam.seed = int(rng.integers(1, 1 << 62))
                 beam.score_fluence = 1
                 eng._ctx.mc_run(beam)
-                eng._ctx.sur_targets(beam, origin, ds.t.view(i, (T, T, T)), ds.w.view(i, (T, T, T)))
-                ds.meta.append({'theta': math.radians(th), 'spot': spot, 'power': P, 'epoch': ep + 1,
-                                'front': origin[3]})
-                i += 1
-                if progress and i % progress == 0:
+                for k, o in enumerate(ws):
+                    eng._ctx.sur_targets(beam, o, ds.t.view(i + k, (T, T, T)), ds.w.view(i + k, (T, T, T)))
+                    ds.meta.append({'theta': math.radians(th), 'spot': spot, 'power': P, 'epoch': ep + 1,
+                                    'front': o[3], 'window': k})
+                before = i
+                i += len(ws)
+                if progress and i // progress > before // progress:
                     eng._ctx.sync()
                     print(f"surrogate coupled data: {i}/{n} windows", flush=True)
                 if i == n:

@@ -85,11 +85,25 @@ int bd_fno_sync(bd_fno *f);
  * on the axis; origin[3] receives that voxel's z index (Nz if none). */
 int bd_sur_window(bd_ctx *ctx, const bd_beam *beam, int margin, int32_t *origin);
 
+/* The tiled window set: the window of bd_sur_window, then windows stacked
+ * above it along the beam axis, each overlapping the one below by 8 voxels,
+ * until one reaches 8 voxels above the tissue surface around the crater (the
+ * first non-void z over nine columns within +-28 voxels of the axis) or the
+ * domain top; at most max_windows (1..4).  origins[4k..4k+3] as in
+ * bd_sur_window, bottom window first; the count goes to *n_windows. */
+int bd_sur_windows(bd_ctx *ctx, const bd_beam *beam, int margin, int max_windows, int32_t *origins,
+                   int *n_windows);
+
 /* Input features of the window (x_dev: float32 [64][64][64][8], channels
  * log1p(mu_a), log1p(mu_s/10), g, n-1, ballistic beam log-intensity,
  * sin(theta), spot/0.5mm - 1, ln(1 + optical depth along the beam)/ln(101))
- * from the packed transport optics. */
+ * from the packed transport optics.  The ballistic march starts at the
+ * domain top, so channels 4 and 7 include what lies above the window. */
 int bd_sur_features(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, float *x_dev);
+
+/* The same for n windows (origins as from bd_sur_windows) into
+ * x_dev [n][64][64][64][8], in one launch. */
+int bd_sur_features_n(bd_ctx *ctx, const bd_beam *beam, int n, const int32_t *origins, float *x_dev);
 
 /* Training target t = ln(Phi/E0 + 1e-4) and loss weights (sum one) of the
  * window from the context's last MC tallies (bd_mc_run with score_fluence). */
@@ -101,6 +115,14 @@ int bd_sur_targets(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, floa
  * elsewhere.  into_q != 0 writes the context's Q (then run bd_epoch_q);
  * into_q == 0 writes a separate field that bd_sur_error compares with Q. */
 int bd_sur_apply(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, const float *y_dev, int into_q);
+
+/* The same over n windows (y_dev [n][64][64][64]): per voxel the ramp-weighted
+ * mean of the windows holding it, sum_k r_k q_k / max(W, 1), plus
+ * (1 - min(W, 1)) of the kept field, W = sum_k r_k (r_k: the 8-voxel face
+ * ramps of window k at its faces inside the domain; overlapping ramps sum
+ * to one). */
+int bd_sur_apply_n(bd_ctx *ctx, const bd_beam *beam, int n, const int32_t *origins, const float *y_dev,
+                   int into_q);
 
 /* Keep the current Q (an MC epoch's field) as the field outside the window. */
 int bd_sur_keep_stale(bd_ctx *ctx);

@@ -44,27 +44,52 @@ def fresnel_u(n1, n2, ci):
     return out
 
 
-def window(mu_a, mu_s, ablated, spacing, xc, yc, theta, phi=0.0, margin=12):
-    """Window origin and the first non-void z on the beam axis."""
+def _first_solid(mu_a, mu_s, ablated, X, Y):
+    """First z whose voxel (X(z), Y) is intact and not void, or None."""
+    Nx, Ny, Nz = mu_a.shape
+    for z in range(Nz):
+        Xz = X(z)
+        if 0 <= Xz < Nx and 0 <= Y < Ny and not ablated[Xz, Y, z] and mu_a[Xz, Y, z] + mu_s[Xz, Y, z] > 0:
+            return z
+    return None
+
+
+def windows(mu_a, mu_s, ablated, spacing, xc, yc, theta, phi=0.0, margin=12, max_windows=4):
+    """Tiled window origins (bottom first), the front z on the beam axis and the
+    surface z (first non-void z over nine probe columns around the axis)."""
     Nx, Ny, Nz = mu_a.shape
     dx, dy, dz = spacing
     sdir = 1 if math.cos(phi) >= 0 else -1
     tan0 = math.tan(theta)
     Y = int(math.floor(yc / dy))
-    front = Nz
-    for z in range(Nz):
-        X = int(math.floor((xc + sdir * (z + 0.5) * dz * tan0) / dx))
-        if not (0 <= X < Nx and 0 <= Y < Ny):
-            continue
-        if not ablated[X, Y, z] and mu_a[X, Y, z] + mu_s[X, Y, z] > 0:
-            front = z
-            break
-    z0 = min(front - margin, Nz - SW)
-    z0 = max(z0, 0)
-    xmid = (xc + sdir * (z0 + 0.5 * SW) * dz * tan0) / dx
-    x0 = max(min(int(round(xmid - 0.5 * SW)), Nx - SW), 0)
+
+    def axis_x(z):
+        return int(math.floor((xc + sdir * (z + 0.5) * dz * tan0) / dx))
+
+    front = _first_solid(mu_a, mu_s, ablated, axis_x, Y)
+    front = Nz if front is None else front
+    surf = front
+    for ox, oy in ((16, 0), (-16, 0), (0, 16), (0, -16), (28, 0), (-28, 0), (0, 28), (0, -28)):
+        z = _first_solid(mu_a, mu_s, ablated, lambda zz: axis_x(zz) + ox, Y + oy)
+        if z is not None:
+            surf = min(surf, z)
     y0 = max(min(int(round(yc / dy - 0.5 * SW)), Ny - SW), 0)
-    return (x0, y0, z0), front
+    z0 = max(min(front - margin, Nz - SW), 0)
+    out = []
+    while True:
+        xmid = (xc + sdir * (z0 + 0.5 * SW) * dz * tan0) / dx
+        x0 = max(min(int(round(xmid - 0.5 * SW)), Nx - SW), 0)
+        out.append((x0, y0, z0))
+        if z0 <= 0 or z0 <= surf - RAMP or len(out) == max_windows:
+            break
+        z0 = max(z0 - (SW - RAMP), 0)
+    return out, front, surf
+
+
+def window(mu_a, mu_s, ablated, spacing, xc, yc, theta, phi=0.0, margin=12):
+    """Window origin and the first non-void z on the beam axis."""
+    w, front, _ = windows(mu_a, mu_s, ablated, spacing, xc, yc, theta, phi, margin, max_windows=1)
+    return w[0], front
 
 
 def _gather(a, origin, fill):
@@ -77,33 +102,37 @@ def _gather(a, origin, fill):
     return out
 
 
+FUP = 192   # columns marched up-beam of the window
+
+
 def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0, with_tau=False):
-    """Collimated beam intensity (relative to its peak) over the window, and
-    with_tau: the optical depth along the beam from the window top."""
+    """Collimated beam intensity (relative to its peak) over the window, marched
+    from the domain top, and with_tau: the optical depth along the beam from the
+    domain top.  The march covers the window's columns and FUP columns up-beam."""
     Nx, Ny, Nz = mu_a.shape
     dx, dy, dz = spacing
     x0, y0, z0 = origin
     sdir = 1 if math.cos(phi) >= 0 else -1
     s0 = n_amb * math.sin(theta)
     w0 = spot / 2.0
-    X = np.arange(x0 - SW, x0 + SW)
+    xlo = x0 - FUP if sdir > 0 else x0
+    tw = x0 - xlo
+    X = np.arange(xlo, xlo + SW + FUP)
     Y = np.arange(y0, y0 + SW)
-    xa_top = xc + sdir * z0 * dz * math.tan(theta)
-    u = ((X + 0.5) * dx - xa_top) * math.cos(theta)
+    u = ((X + 0.5) * dx - xc) * math.cos(theta)
     v = (Y + 0.5) * dy - yc
-    Bm = np.exp(-2.0 * (u[:, None] ** 2 + v[None, :] ** 2) / (w0 * w0))   # [128, 64]
+    Bm = np.exp(-2.0 * (u[:, None] ** 2 + v[None, :] ** 2) / (w0 * w0))
     inx = (X >= 0) & (X < Nx)
     iny = (Y >= 0) & (Y < Ny)
     nprev = np.full(Bm.shape, n_amb)
     tau = np.zeros(Bm.shape)
     out = np.zeros((SW, SW, SW))
     tout = np.zeros((SW, SW, SW))
-    for k in range(SW):
-        Z = z0 + k
+    for Z in range(z0 + SW):
         ma = np.zeros(Bm.shape)
         ms = np.zeros(Bm.shape)
         nk = np.full(Bm.shape, n_amb)
-        if 0 <= Z < Nz:
+        if Z < Nz:
             m = inx[:, None] & iny[None, :]
             xi = np.clip(X, 0, Nx - 1)[:, None]
             yi = np.clip(Y, 0, Ny - 1)[None, :]
@@ -131,8 +160,10 @@ def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_am
         tau = np.where(ok, (1.0 - sh) * tau + sh * Tn + (ma + ms) * dz / np.where(ok, ck, 1.0), tau + (ma + ms) * dz)
         Bm = Bk
         nprev = nk
-        out[:, :, k] = Bm[SW:, :]
-        tout[:, :, k] = tau[SW:, :]
+        k = Z - z0
+        if k >= 0:
+            out[:, :, k] = Bm[tw:tw + SW, :]
+            tout[:, :, k] = tau[tw:tw + SW, :]
     return (out, tout) if with_tau else out
 
 
@@ -177,6 +208,30 @@ def _ramp(lo, hi):
     if hi:
         r = np.minimum(r, (SW - i) / RAMP)
     return r
+
+
+def apply_n(ys, mu_a, Qs, origins, power, spot):
+    """Composed absorbed-power field over several windows (float64): per voxel
+    sum_k r_k q_k / max(W, 1) + (1 - min(W, 1)) Qs with W = sum_k r_k."""
+    Nx, Ny, Nz = mu_a.shape
+    w0 = spot / 2.0
+    E0 = power / (math.pi * w0 * w0 / 2.0)
+    n = (Nx, Ny, Nz)
+    S = np.zeros(mu_a.shape)
+    W = np.zeros(mu_a.shape)
+    for y, o in zip(ys, origins):
+        x0, y0, z0 = o[:3]
+        rs = [_ramp(o[d] > 0, o[d] + SW < n[d]) for d in range(3)]
+        r = rs[0][:, None, None] * rs[1][None, :, None] * rs[2][None, None, :]
+        ex, ey, ez = min(SW, Nx - x0), min(SW, Ny - y0), min(SW, Nz - z0)
+        sl = (slice(x0, x0 + ex), slice(y0, y0 + ey), slice(z0, z0 + ez))
+        q = E0 * mu_a[sl] * np.maximum(np.exp(y[:ex, :ey, :ez]) - PHI_EPS, 0.0)
+        S[sl] += r[:ex, :ey, :ez] * q
+        W[sl] += r[:ex, :ey, :ez]
+    out = np.array(Qs, dtype=np.float64, copy=True)
+    m = W > 0
+    out[m] = S[m] / np.maximum(W[m], 1.0) + (1.0 - np.minimum(W[m], 1.0)) * out[m]
+    return out
 
 
 def apply(y, mu_a, Qs, origin, power, spot):

@@ -84,6 +84,7 @@ def test_window_features_apply_and_error_on_a_larger_domain():
     assert tuple(o[:3]) == oo and o[3] == front
     x = DeviceArray((64, 64, 64, 8))
     c.sur_features(beam, o, x)
+    c.sync()
     xo = S.features(*(_f32(props[k]) for k in ('mu_a', 'mu_s', 'g', 'n')), SP, oo, xc, yc, th, 0.5)
     np.testing.assert_allclose(x.download(), xo, rtol=2e-5, atol=2e-5)
     # composition over a stale field, into Q and into the judged field
@@ -98,6 +99,54 @@ def test_window_features_apply_and_error_on_a_larger_domain():
     ew = S.rel_error(_f32(Qs), want, abl)
     assert e[0] == pytest.approx(ew[0], rel=1e-5) and e[1] == pytest.approx(ew[1], rel=1e-5)
     c.sur_apply(beam, o, yd, into_q=True)
+    np.testing.assert_allclose(c.download('Q'), want, rtol=2e-6, atol=1e-6)
+    c.close()
+
+
+@pytest.mark.parametrize("phi_deg", [0.0, 180.0])
+def test_tiled_windows_features_and_apply_match_oracle(phi_deg):
+    """A deep oblique crater: the stacked windows, their features marched from
+    the domain top (towards +x and towards -x), and the composition over
+    overlapping windows, against the oracle."""
+    from simulation._bdlib import DeviceArray
+    from simulation.voxel_mc_3d import make_beam
+    rng = np.random.default_rng(9)
+    shape = (144, 80, 200)
+    props = _stack(shape, rng)
+    th = math.radians(22.0)
+    sd = 1 if phi_deg == 0.0 else -1
+    xc, yc = (40.0 if sd > 0 else 104.0) * 0.015, 40.5 * 0.015
+    # a crater following the beam axis down to z = 150
+    X, Y, Z = np.meshgrid(np.arange(shape[0]) + 0.5, np.arange(shape[1]) + 0.5, np.arange(shape[2]) + 0.5,
+                          indexing='ij')
+    ax = xc / 0.015 + sd * Z * math.tan(th)
+    abl = ((X - ax) ** 2 + (Y - yc / 0.015) ** 2 < 9.0 ** 2) & (Z >= 21) & (Z < 150)
+    for k in ('mu_a', 'mu_s', 'g'):
+        props[k][abl] = 0.0
+    props['n'][abl] = 1.0
+    c = _ctx(props, shape)
+    c.upload('ABLATED', abl)
+    beam = make_beam(40.0, 100_000, 0.5, 22.0, phi_deg, (xc, yc, 0.0), 0.0, 0, 'gaussian', 7)
+    ws = c.sur_windows(beam, 12, 4)
+    wo, front, surf = S.windows(_f32(props['mu_a']), _f32(props['mu_s']), abl, SP, xc, yc, th,
+                                phi=math.radians(phi_deg), margin=12)
+    assert [w[:3] for w in ws] == wo and all(w[3] == front for w in ws)
+    assert len(ws) >= 3
+    x = DeviceArray((len(ws), 64, 64, 64, 8))
+    c.sur_features_n(beam, ws, x)
+    c.sync()
+    xg = x.download()
+    for k, o in enumerate(wo):
+        xo = S.features(*(_f32(props[n]) for n in ('mu_a', 'mu_s', 'g', 'n')), SP, o, xc, yc, th, 0.5,
+                        phi=math.radians(phi_deg))
+        np.testing.assert_allclose(xg[k], xo, rtol=2e-5, atol=2e-5, err_msg=f"window {k}")
+    Qs = rng.uniform(0.0, 50.0, shape).astype(np.float32)
+    c.upload('Q', Qs)
+    c.sur_keep_stale()
+    ys = rng.normal(-3.0, 1.0, (len(ws), 64, 64, 64)).astype(np.float32)
+    yd = DeviceArray(ys.shape).upload(ys)
+    c.sur_apply_n(beam, ws, yd, into_q=True)
+    want = S.apply_n([_f32(y) for y in ys], _f32(props['mu_a']), _f32(Qs), wo, 40.0, 0.5)
     np.testing.assert_allclose(c.download('Q'), want, rtol=2e-6, atol=1e-6)
     c.close()
 

@@ -81,3 +81,52 @@ def test_window_finds_the_front():
     o, front = S.window(ma, ms, abl, SP, 64 * 0.015, 48 * 0.015, math.radians(22.0), margin=12)
     assert front == 40 and o[2] == 28
     assert o[1] == 16
+
+
+def test_march_from_the_domain_top_is_beer_lambert_below_the_window_top():
+    # a window 40 planes down sees the attenuation of everything above it
+    ma, ms, g, n = _slab(shape=(64, 64, 128))
+    xc = yc = 32 * 0.015
+    B = S.ballistic(ma, ms, n, SP, (0, 0, 40), xc, yc, 0.0, 0.5)
+    R = ((1.0 - 1.4) / 2.4) ** 2
+    I0 = math.exp(-2.0 * (2 * (0.5 * 0.015) ** 2) / 0.25 ** 2)
+    k = np.arange(64)
+    np.testing.assert_allclose(B[32, 32, :], I0 * (1 - R) * np.exp(-(0.5 + 3.0) * 0.015 * (40 + k + 1)), rtol=1e-12)
+    _, tau = S.ballistic(ma, ms, n, SP, (0, 0, 40), xc, yc, 0.0, 0.5, with_tau=True)
+    np.testing.assert_allclose(tau[32, 32, :], (0.5 + 3.0) * 0.015 * (40 + k + 1), rtol=1e-12)
+
+
+def test_windows_tile_the_crater_from_front_to_surface():
+    ma, ms, g, n = _slab(air=21, shape=(128, 96, 200))
+    abl = np.zeros(ma.shape, bool)
+    abl[54:74, 38:58, 21:150] = True          # a 20-voxel-wide crater down to z = 150
+    ws, front, surf = S.windows(ma, ms, abl, SP, 64.5 * 0.015, 48.5 * 0.015, 0.0, margin=12)
+    assert front == 150 and surf == 21
+    assert [w[2] for w in ws] == [136, 80, 24, 0]
+    assert all(w[:2] == (32, 16) for w in ws)
+    # consecutive windows overlap by at least the ramp; the stack reaches above the surface
+    assert all(a[2] - b[2] <= S.SW - S.RAMP for a, b in zip(ws, ws[1:]))
+    assert ws[-1][2] <= surf - S.RAMP
+    # a shallow front needs one window
+    abl[:] = False
+    ws, front, surf = S.windows(ma, ms, abl, SP, 64.5 * 0.015, 48.5 * 0.015, 0.0, margin=12)
+    assert front == surf == 21 and [w[2] for w in ws] == [9]
+
+
+def test_apply_n_is_a_partition_of_unity_in_the_overlap():
+    rng = np.random.default_rng(2)
+    shape = (80, 70, 150)
+    ma = rng.uniform(0.1, 2.0, shape)
+    Qs = rng.uniform(0.0, 10.0, shape)
+    F = rng.uniform(1.0, 20.0, shape)            # one consistent field predicted by both windows
+    E0 = 40.0 / (math.pi * 0.25 ** 2 / 2)
+    ws = [(8, 3, 80), (8, 3, 24)]                # overlap z in [80, 88)
+    ys = [np.log(F[o[0]:o[0] + 64, o[1]:o[1] + 64, o[2]:o[2] + 64] / (E0 * ma[o[0]:o[0] + 64, o[1]:o[1] + 64,
+                                                                            o[2]:o[2] + 64]) + 1e-4) for o in ws]
+    out = S.apply_n(ys, ma, Qs, ws, 40.0, 0.5)
+    inner = (slice(16, 64), slice(11, 59), slice(32, 136))   # away from the outer faces
+    np.testing.assert_allclose(out[inner], F[inner], rtol=1e-10)
+    assert out[0, 0, 0] == Qs[0, 0, 0] and out[79, 69, 149] == Qs[79, 69, 149]
+    # one window reduces to apply()
+    np.testing.assert_allclose(S.apply_n(ys[:1], ma, Qs, ws[:1], 40.0, 0.5), S.apply(ys[0], ma, Qs, ws[0], 40.0, 0.5),
+                               rtol=1e-12)
