@@ -95,6 +95,22 @@ def run_plain(kw, steps, warmup, graph):
     return out
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_substep_traffic_layered_256x256x192.md")
+
+
+def measured_traffic():
+    """HBM bytes per one-step sub-step launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("- traffic_bytes_per_launch:"):
+                    return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
 def run_surrogate(kw, steps, warmup, weights):
     import numpy as np
     from simulation.coupled_ablation import CoupledAblationEngine
@@ -192,8 +208,8 @@ def main():
             "workload": "coupled layered stack: 256x256x192 voxels of 15 um, 7 layers + 21 voxels ambient air, "
                         "40 W at 22 deg, 0.5 mm spot, 1 ms epochs of 150k photons + 20 x 50 us sub-steps, boil "
                         "100 C, ablate 350 C, Arrhenius char, diffusion plume on 128^3" +
-                        ("" if args.plain_only else "; surrogate on 64^3 beam window, MC judge every 10th epoch, "
-                                                    "fallback above 5% L2"),
+                        ("" if args.plain_only else "; surrogate on up to four 64^3 windows tiled along the beam, "
+                                                    "MC judge every 10th epoch, fallback above 5% L2"),
             "voxels": p["nvox"], "photons_per_epoch": p["n_photons"], "substeps_per_epoch": p["n_sub"],
             "mode": "plain MC" if args.plain_only else "surrogate+MC",
         },
@@ -212,7 +228,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if n_pair else measured_traffic(),
             "bytes_per_voxel": SUBSTEP_BYTES_PER_VOXEL,
             "avg_launch_ms": round(sub_launch_ms, 5),
             "launches_per_epoch": n_launch,
