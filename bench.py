@@ -152,7 +152,30 @@ def main():
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--copy-kernel", action="store_true",
                     help="only run three 256 MiB device-to-device copies (calibrates the HBM byte counters)")
+    ap.add_argument("--mc-probe", default=None, metavar="N1,N2,...",
+                    help="only time the transport kernel on the workload's state after three epochs, at these "
+                         "photon counts (three launches each; for profiling the MC kernel)")
     args = ap.parse_args()
+    if args.mc_probe:
+        from simulation.coupled_ablation import CoupledAblationEngine
+        from simulation.workloads import layered_stack
+        eng = CoupledAblationEngine(**layered_stack())
+        for _ in range(3):
+            eng.step_epoch()
+        beam = eng._epoch_beam()
+        out = {}
+        for n in (int(v) for v in args.mc_probe.split(",")):
+            beam.n_photons = n
+            ts = []
+            for rep in range(3):
+                beam.seed = 77 + rep
+                ts.append(eng._ctx.mc_run(beam).kernel_ms)
+            out[n] = {"kernel_ms_min": round(min(ts), 3), "kernel_ms_max": round(max(ts), 3),
+                      "photons_per_s": round(n / (min(ts) * 1e-3), 1)}
+            log(f"bench: mc probe {n} photons done")
+        eng.close()
+        print(json.dumps({"mc_probe": out}))
+        return
     if args.copy_kernel:
         from simulation import _bdlib as B
         n = 256 * 1024 * 1024
@@ -162,8 +185,9 @@ def main():
         print(json.dumps({"copy_bytes_each_way": n, "copies": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
-                                   "(the multi-GPU sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus above 1 is not supported "
+                                   "(a run would shard the domain in x slabs with a one-plane halo per "
+                                   "sub-step and an all-reduce of the MC tallies)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np

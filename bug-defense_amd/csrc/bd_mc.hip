@@ -295,19 +295,30 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
             const int v = (p.ix * Ny + p.iy) * Nz + p.iz;
             const float mua = p.o.x;
             const float mut = p.o.x + p.o.y;
-            // _dist_to_voxel_exit (voxel_mc_3d.py:72-98)
-            const float sx = p.ux > 1e-15f ? ((p.ix + 1) * a.dx - p.x) / p.ux
-                           : (p.ux < -1e-15f ? (p.ix * a.dx - p.x) / p.ux : 1e30f);
-            const float sy = p.uy > 1e-15f ? ((p.iy + 1) * a.dy - p.y) / p.uy
-                           : (p.uy < -1e-15f ? (p.iy * a.dy - p.y) / p.uy : 1e30f);
-            const float sz = p.uz > 1e-15f ? ((p.iz + 1) * a.dz - p.z) / p.uz
-                           : (p.uz < -1e-15f ? (p.iz * a.dz - p.z) / p.uz : 1e30f);
+            // _dist_to_voxel_exit (voxel_mc_3d.py:72-98), with the hardware reciprocal
+            // of the direction (1 ulp; the reference runs this under numba fastmath)
+            const float sx = p.ux > 1e-15f ? ((p.ix + 1) * a.dx - p.x) * __builtin_amdgcn_rcpf(p.ux)
+                           : (p.ux < -1e-15f ? (p.ix * a.dx - p.x) * __builtin_amdgcn_rcpf(p.ux) : 1e30f);
+            const float sy = p.uy > 1e-15f ? ((p.iy + 1) * a.dy - p.y) * __builtin_amdgcn_rcpf(p.uy)
+                           : (p.uy < -1e-15f ? (p.iy * a.dy - p.y) * __builtin_amdgcn_rcpf(p.uy) : 1e30f);
+            const float sz = p.uz > 1e-15f ? ((p.iz + 1) * a.dz - p.z) * __builtin_amdgcn_rcpf(p.uz)
+                           : (p.uz < -1e-15f ? (p.iz * a.dz - p.z) * __builtin_amdgcn_rcpf(p.uz) : 1e30f);
             int axis;
             float s_exit;
             if (sx <= sy && sx <= sz) { axis = 0; s_exit = sx; }
             else if (sy <= sz) { axis = 1; s_exit = sy; }
             else { axis = 2; s_exit = sz; }
             s_exit = fmaxf(s_exit, 0.f);
+            // optics of the exit voxel, loaded before the interaction test so that the
+            // load's latency overlaps the rest of the event (unused if the photon
+            // interacts inside the voxel)
+            int qx = p.ix, qy = p.iy, qz = p.iz;
+            if (axis == 0) qx += p.ux > 0.f ? 1 : -1;
+            else if (axis == 1) qy += p.uy > 0.f ? 1 : -1;
+            else qz += p.uz > 0.f ? 1 : -1;
+            float4 o_next = make_float4(0.f, 0.f, 0.f, 1.f);
+            if (qx >= 0 && qx < Nx && qy >= 0 && qy < Ny && qz >= 0 && qz < Nz)
+                o_next = a.opt[((int64_t)qx * Ny + qy) * Nz + qz];
             t_ev += 1;
             p.ev++;
             if (mut > 1e-12f && mut * s_exit >= p.tau) {
@@ -356,8 +367,9 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
                     p.uz = ct * sg;
                 } else {
                     const float den = sqrtf(fmaxf(1.f - p.uz * p.uz, 1e-12f));
-                    const float nx = st * (p.ux * p.uz * cp - p.uy * sp) / den + p.ux * ct;
-                    const float ny = st * (p.uy * p.uz * cp + p.ux * sp) / den + p.uy * ct;
+                    const float iden = __builtin_amdgcn_rcpf(den);
+                    const float nx = st * (p.ux * p.uz * cp - p.uy * sp) * iden + p.ux * ct;
+                    const float ny = st * (p.uy * p.uz * cp + p.ux * sp) * iden + p.uy * ct;
                     const float nz = -st * cp * den + p.uz * ct;
                     p.ux = nx;
                     p.uy = ny;
@@ -365,9 +377,10 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
                 }
                 const float nrm = sqrtf(p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
                 if (nrm > 1e-12f) {
-                    p.ux /= nrm;
-                    p.uy /= nrm;
-                    p.uz /= nrm;
+                    const float inrm = __builtin_amdgcn_rcpf(nrm);
+                    p.ux *= inrm;
+                    p.uy *= inrm;
+                    p.uz *= inrm;
                 }
                 p.tau = -__logf(fmaxf(rng_u(rng), 1e-12f));
             } else {
@@ -431,7 +444,7 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
                         continue;
                     }
                 }
-                const float4 o2 = a.opt[((int64_t)jx * Ny + jy) * Nz + jz];
+                const float4 o2 = o_next;  // (jx, jy, jz) == (qx, qy, qz), inside the domain here
                 const float n1 = p.o.w, n2 = o2.w;
                 if (fabsf(n1 - n2) > 1e-9f) {
                     float &un = axis == 0 ? p.ux : (axis == 1 ? p.uy : p.uz);
