@@ -11,6 +11,8 @@
 // The host waits once for the diagnostics at the end (callers receive n_new).
 #include "bd_internal.hpp"
 
+#include <vector>
+
 #include <cmath>
 
 namespace bd {
@@ -156,13 +158,11 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
 namespace bd {
 void plume_diag_from(const double *h, double out[4]);
 
-static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
+// after the transport: the absorbed-power total, or the cleared fields of a
+// laser-off epoch
+static int enqueue_after_mc(Ctx *c, const bd_beam *beam)
 {
-    const bd_physics &p = c->phys;
-    const bool plume = c->plume.enabled != 0;
-    const bool diff_plume = plume && c->plume.model == 0;
     if (beam->power > 0.0) {
-        BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2));
         BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
     } else {
         BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
@@ -170,6 +170,15 @@ static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
         BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
         BD_TRY(launch_fill(c, c->red + 40, K_F64, 1, 0.0));
     }
+    return BD_OK;
+}
+
+// everything after the absorbed power: sub-steps, plume, phase update, readback
+static int enqueue_epoch_tail(Ctx *c, int n_substeps)
+{
+    const bd_physics &p = c->phys;
+    const bool plume = c->plume.enabled != 0;
+    const bool diff_plume = plume && c->plume.model == 0;
     BD_HIP(hipEventRecordWithFlags(c->gev[2], c->stream, hipEventRecordExternal));
     if (diff_plume) BD_TRY(launch_plume_void(c));
     int flags = BD_SUB_PLATEAU | BD_SUB_MASK_Q | BD_SUB_PEAK;
@@ -198,6 +207,13 @@ static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
     return BD_OK;
 }
 
+static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
+{
+    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2));
+    BD_TRY(enqueue_after_mc(c, beam));
+    return enqueue_epoch_tail(c, n_substeps);
+}
+
 static bool same_beam(const bd_beam &a, const bd_beam &b)
 {
     return a.power == b.power && a.spot_dia == b.spot_dia && a.theta_rad == b.theta_rad && a.phi_rad == b.phi_rad &&
@@ -209,22 +225,24 @@ static bool same_beam(const bd_beam &a, const bd_beam &b)
 
 }  // namespace bd
 
-extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_before,
-                                     int64_t epoch, uint64_t seed_base)
+namespace bd {
+// checks and buffers shared by the graph and gang epochs (no host round trip inside)
+static int prepare_enqueued_epoch(Ctx *c, int n_substeps, const char *who)
 {
-    BD_CHECK(h && beam, "bd_epoch_graph: null argument");
-    BD_CHECK(n_substeps >= 1, "bd_epoch_graph: n_substeps must be >= 1");
-    Ctx *c = &h->c;
     if (!c->has_phys) {
-        set_error("bd_epoch_graph: bd_set_physics has not been called");
+        set_error(std::string(who) + ": bd_set_physics has not been called");
         return BD_ESTATE;
     }
     if (c->plume.enabled && (c->plume.model != 0 || c->plume.blast_expansion)) {
-        set_error("bd_epoch_graph: the blast / separable plume needs host round trips; use bd_epoch");
+        set_error(std::string(who) + ": the blast / separable plume needs host round trips; use bd_epoch");
         return BD_ESTATE;
     }
     if (n_substeps % 2) {
-        set_error("bd_epoch_graph: an odd sub-step count leaves the ping-pong buffers swapped; use bd_epoch");
+        set_error(std::string(who) + ": an odd sub-step count leaves the ping-pong buffers swapped; use bd_epoch");
+        return BD_ESTATE;
+    }
+    if (c->graph_pending) {
+        set_error(std::string(who) + ": the previous epoch was not finished");
         return BD_ESTATE;
     }
     BD_TRY(ensure_state(c));
@@ -232,6 +250,29 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
     BD_TRY(ensure(c, (void **)&c->Tpeak, (size_t)c->nvox * 4));
     BD_TRY(ensure(c, (void **)&c->to_remove, (size_t)c->nvox));
     if (c->phys.has_arrhenius) BD_TRY(ensure(c, (void **)&c->Om, (size_t)c->nvox * 4));
+    return BD_OK;
+}
+
+// per-epoch scalars -> device, on the context stream (pinned staging in host_pinned)
+static int upload_epoch_params(Ctx *c, int n_substeps, double t_before, int64_t epoch, uint64_t seed_base)
+{
+    EpochParams *hp = (EpochParams *)((char *)c->host_pinned + 56 * sizeof(double));
+    hp->seed_base = seed_base;
+    hp->epoch = (long long)epoch;
+    hp->t = t_before;
+    hp->dt_epoch = n_substeps * c->phys.dt_sub;
+    BD_HIP(hipMemcpyAsync(c->ep, hp, sizeof(EpochParams), hipMemcpyHostToDevice, c->stream));
+    return BD_OK;
+}
+}  // namespace bd
+
+extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_before,
+                                     int64_t epoch, uint64_t seed_base)
+{
+    BD_CHECK(h && beam, "bd_epoch_graph: null argument");
+    BD_CHECK(n_substeps >= 1, "bd_epoch_graph: n_substeps must be >= 1");
+    Ctx *c = &h->c;
+    BD_TRY(prepare_enqueued_epoch(c, n_substeps, "bd_epoch_graph"));
     const bool recapture = !c->graph_exec || c->graph_nsub != n_substeps || !same_beam(c->graph_beam, *beam) ||
                            c->graph_T != c->T || c->graph_P != c->P || c->graph_H != c->H ||
                            c->graph_F != c->flags;
@@ -278,23 +319,51 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
         c->graph_nsub = n_substeps;
     }
     // per-epoch scalars -> device (outside the graph), replay; the readback is bd_epoch_graph_finish
-    if (c->graph_pending) {
-        set_error("bd_epoch_graph_launch: the previous epoch graph was not finished");
-        return BD_ESTATE;
-    }
     const double dt_epoch = n_substeps * c->phys.dt_sub;
-    EpochParams *hp = (EpochParams *)((char *)c->host_pinned + 56 * sizeof(double));
-    hp->seed_base = seed_base;
-    hp->epoch = (long long)epoch;
-    hp->t = t_before;
-    hp->dt_epoch = dt_epoch;
     BD_HIP(hipEventRecord(c->ev2, c->stream));
-    BD_HIP(hipMemcpyAsync(c->ep, hp, sizeof(EpochParams), hipMemcpyHostToDevice, c->stream));
+    BD_TRY(upload_epoch_params(c, n_substeps, t_before, epoch, seed_base));
     BD_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     BD_HIP(hipEventRecord(c->ev3, c->stream));
     c->graph_pending = true;
     c->pending_beam = *beam;
     c->pending_dt_epoch = dt_epoch;
+    return BD_OK;
+}
+
+// One epoch of n contexts whose photon transport runs as a single launch
+// (k_mc_gang); every other step is queued on each context's own stream, as in
+// the graph epoch.  Collect each context with bd_epoch_graph_finish.
+extern "C" int bd_epoch_gang_launch(bd_ctx *const *hs, int n, const bd_beam *beams, int n_substeps,
+                                    const double *t_before, const int64_t *epochs, const uint64_t *seed_bases)
+{
+    BD_CHECK(hs && beams && t_before && epochs && seed_bases, "bd_epoch_gang_launch: null argument");
+    BD_CHECK(n >= 1 && n <= 4096, "bd_epoch_gang_launch: n must be in [1, 4096]");
+    BD_CHECK(n_substeps >= 1, "bd_epoch_gang_launch: n_substeps must be >= 1");
+    std::vector<Ctx *> on;
+    std::vector<const bd_beam *> on_beams;
+    for (int i = 0; i < n; ++i) {
+        BD_CHECK(hs[i], "bd_epoch_gang_launch: null context");
+        for (int j = 0; j < i; ++j) BD_CHECK(hs[j] != hs[i], "bd_epoch_gang_launch: a context appears twice");
+        Ctx *c = &hs[i]->c;
+        BD_TRY(prepare_enqueued_epoch(c, n_substeps, "bd_epoch_gang_launch"));
+        BD_TRY(launch_substeps(c, 0, 1, BD_SUB_PLATEAU, true));  // tile tuning (first epoch) before any enqueue
+        BD_HIP(hipEventRecord(c->ev2, c->stream));
+        BD_TRY(upload_epoch_params(c, n_substeps, t_before[i], epochs[i], seed_bases[i]));
+        if (beams[i].power > 0.0) {
+            on.push_back(c);
+            on_beams.push_back(&beams[i]);
+        }
+    }
+    BD_TRY(launch_mc_gang(on.data(), on_beams.data(), (int)on.size()));
+    for (int i = 0; i < n; ++i) {
+        Ctx *c = &hs[i]->c;
+        BD_TRY(enqueue_after_mc(c, &beams[i]));
+        BD_TRY(enqueue_epoch_tail(c, n_substeps));
+        BD_HIP(hipEventRecord(c->ev3, c->stream));
+        c->graph_pending = true;
+        c->pending_beam = beams[i];
+        c->pending_dt_epoch = n_substeps * c->phys.dt_sub;
+    }
     return BD_OK;
 }
 

@@ -158,6 +158,11 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out);
 // enqueue only (no host wait); with ep != null the photon key comes from the device params
 // timed: 0 no events, 1 ev0/ev1 around the transport kernel, 2 graph-external gev[0]/gev[1]
 int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed);
+// One transport launch for n contexts (each with power on, EpochParams in c->ep):
+// prepares each on its own stream, traces all photons in one kernel on a
+// library stream, then queues each context's absorbed-power conversion on its
+// stream after it.
+int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n);
 int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J);
 int launch_phase_enqueue(Ctx *c, double t_now, const EpochParams *ep);
 int launch_plume_diag_enqueue(Ctx *c);

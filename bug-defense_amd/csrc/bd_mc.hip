@@ -142,7 +142,7 @@ struct Photon {
     long long ev;
 };
 
-__global__ __launch_bounds__(256) void k_mc(McArgs a)
+__device__ __forceinline__ void mc_trace(const McArgs &a)
 {
     const int lane = threadIdx.x & 63;
     const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
@@ -481,6 +481,19 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
     }
 }
 
+__global__ __launch_bounds__(256) void k_mc(McArgs a) { mc_trace(a); }
+
+// Several contexts' transport in one launch: block b traces photons of context
+// block_run[b] (blocks are dealt to contexts in proportion to their photon
+// counts, each block to one context), so a context's arguments are uniform
+// over a block and sit in scalar registers as in k_mc.  Blocks of all runs
+// share the machine: the tails of long-photon runs fill with other runs' work.
+__global__ __launch_bounds__(256) void k_mc_gang(const McArgs *__restrict__ args, const int *__restrict__ block_run)
+{
+    const McArgs a = args[block_run[blockIdx.x]];
+    mc_trace(a);
+}
+
 // ------------------------------------------------------------------ optics pack
 // Transport optics per voxel: base optics plus the plume extinction sampled at
 // the voxel centre (coupled_ablation.py:445-450: mu_a += mu_a_plume,
@@ -531,7 +544,9 @@ __global__ void k_q_from_a(const float2 *__restrict__ AL, float *__restrict__ Q,
         Q[i] = AL[i].x * scale;
 }
 
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed)
+// Pack the optics, clear the tallies and fill the kernel arguments of one
+// context's transport (no launch).
+static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a)
 {
     BD_TRY(launch_pack_optics(c, b->add_plume != 0));
     BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
@@ -548,7 +563,7 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
     BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
     BD_TRY(launch_fill(c, c->counter, K_F64, 1, 0.0));
 
-    McArgs a{};
+    a = McArgs{};
     a.opt = c->optics;
     a.AL = c->AL;
     a.acc = c->red + 16;
@@ -595,23 +610,133 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
     a.k1 = (unsigned)(b->seed >> 32);
     a.score_L = b->score_fluence ? 1 : 0;
     a.ep = ep;
+    return BD_OK;
+}
 
+// Blocks for n photons: one photon per lane, at most 8 blocks per CU (idle lanes
+// refill from the photon counter when there are more photons than lanes).
+static int64_t mc_blocks(Ctx *c, long long n_photons)
+{
     int ncu = 256;
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
-    const int64_t want = (b->n_photons + 255) / 256;
+    const int64_t want = (n_photons + 255) / 256;
     int64_t cap = (int64_t)ncu * 8;
     if (c->mc_blocks_per_cu > 0) cap = (int64_t)ncu * c->mc_blocks_per_cu;
-    const int nblk = (int)(want < cap ? (want < 1 ? 1 : want) : cap);
+    return want < cap ? (want < 1 ? 1 : want) : cap;
+}
+
+// Absorbed power from the absorbed-weight tally (after the transport).
+static int mc_post(Ctx *c, const bd_beam *b)
+{
+    const float scale = (float)(b->power / ((double)b->n_photons * c->g.dx * c->g.dy * c->g.dz));
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_q_from_a, dim3(nb), dim3(256), 0, c->stream, c->AL, c->Q, scale, c->nvox);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed)
+{
+    McArgs a;
+    BD_TRY(mc_prepare(c, b, ep, a));
+    const int nblk = (int)mc_blocks(c, b->n_photons);
     if (timed == 1) BD_HIP(hipEventRecord(c->ev0, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[0], c->stream, hipEventRecordExternal));
     hipLaunchKernelGGL(k_mc, dim3(nblk), dim3(256), 0, c->stream, a);
     BD_HIP(hipGetLastError());
     if (timed == 1) BD_HIP(hipEventRecord(c->ev1, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[1], c->stream, hipEventRecordExternal));
-    const float scale = (float)(b->power / ((double)b->n_photons * c->g.dx * c->g.dy * c->g.dz));
-    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
-    hipLaunchKernelGGL(k_q_from_a, dim3(nb), dim3(256), 0, c->stream, c->AL, c->Q, scale, c->nvox);
+    return mc_post(c, b);
+}
+
+// ---- gang transport (k_mc_gang): one launch for several contexts' photons.
+// Library-wide resources per device: a stream of default priority, an event,
+// and the argument / block-map buffers (device and pinned host staging),
+// grown on demand.  Gang calls on one device must not overlap in time (they
+// share these buffers); each call waits for the previous one's uploads.
+namespace {
+struct GangRes {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    McArgs *d_args = nullptr, *h_args = nullptr;
+    int *d_map = nullptr, *h_map = nullptr;
+    int64_t cap_args = 0, cap_map = 0;
+};
+GangRes g_gang[16];
+
+int gang_res(int dev, int64_t n_args, int64_t n_map, GangRes *&out)
+{
+    if (dev < 0 || dev >= 16) {
+        set_error("gang transport: device index out of range");
+        return BD_EINVAL;
+    }
+    GangRes &g = g_gang[dev];
+    if (!g.stream) {
+        BD_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        BD_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    }
+    BD_HIP(hipStreamSynchronize(g.stream));  // the previous call's uploads read the staging
+    if (n_args > g.cap_args) {
+        if (g.d_args) BD_HIP(hipFree(g.d_args));
+        if (g.h_args) BD_HIP(hipHostFree(g.h_args));
+        g.d_args = nullptr;
+        g.h_args = nullptr;
+        BD_HIP(hipMalloc((void **)&g.d_args, sizeof(McArgs) * n_args));
+        BD_HIP(hipHostMalloc((void **)&g.h_args, sizeof(McArgs) * n_args));
+        g.cap_args = n_args;
+    }
+    if (n_map > g.cap_map) {
+        if (g.d_map) BD_HIP(hipFree(g.d_map));
+        if (g.h_map) BD_HIP(hipHostFree(g.h_map));
+        g.d_map = nullptr;
+        g.h_map = nullptr;
+        BD_HIP(hipMalloc((void **)&g.d_map, sizeof(int) * n_map));
+        BD_HIP(hipHostMalloc((void **)&g.h_map, sizeof(int) * n_map));
+        g.cap_map = n_map;
+    }
+    out = &g;
+    return BD_OK;
+}
+}  // namespace
+
+int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n)
+{
+    if (n < 1) return BD_OK;
+    const int dev = cs[0]->device;
+    int64_t nmap = 0;
+    for (int i = 0; i < n; ++i) {
+        if (cs[i]->device != dev) {
+            set_error("gang transport: all contexts must be on one device");
+            return BD_EINVAL;
+        }
+        nmap += mc_blocks(cs[i], beams[i]->n_photons);
+    }
+    if (nmap > 0x7fffffff) {
+        set_error("gang transport: too many blocks");
+        return BD_EINVAL;
+    }
+    GangRes *g = nullptr;
+    BD_TRY(gang_res(dev, n, nmap, g));
+    int64_t m = 0;
+    for (int i = 0; i < n; ++i) {
+        Ctx *c = cs[i];
+        BD_TRY(mc_prepare(c, beams[i], c->ep, g->h_args[i]));
+        const int64_t nb = mc_blocks(c, beams[i]->n_photons);
+        for (int64_t b = 0; b < nb; ++b) g->h_map[m++] = i;
+        BD_HIP(hipEventRecord(c->ev4, c->stream));
+        BD_HIP(hipStreamWaitEvent(g->stream, c->ev4, 0));
+    }
+    BD_HIP(hipMemcpyAsync(g->d_args, g->h_args, sizeof(McArgs) * n, hipMemcpyHostToDevice, g->stream));
+    BD_HIP(hipMemcpyAsync(g->d_map, g->h_map, sizeof(int) * m, hipMemcpyHostToDevice, g->stream));
+    for (int i = 0; i < n; ++i) BD_HIP(hipEventRecord(cs[i]->gev[0], g->stream));
+    hipLaunchKernelGGL(k_mc_gang, dim3((unsigned)m), dim3(256), 0, g->stream, g->d_args, g->d_map);
     BD_HIP(hipGetLastError());
+    for (int i = 0; i < n; ++i) BD_HIP(hipEventRecord(cs[i]->gev[1], g->stream));
+    BD_HIP(hipEventRecord(g->done, g->stream));
+    for (int i = 0; i < n; ++i) {
+        BD_HIP(hipStreamWaitEvent(cs[i]->stream, g->done, 0));
+        BD_TRY(mc_post(cs[i], beams[i]));
+    }
     return BD_OK;
 }
 

@@ -134,6 +134,9 @@ _SIGS = {
     "bd_epoch_graph_launch": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.c_double,
                                ctypes.c_int64, ctypes.c_uint64], ctypes.c_int),
     "bd_epoch_graph_finish": ([ctypes.c_void_p, ctypes.POINTER(EpochDiag)], ctypes.c_int),
+    "bd_epoch_gang_launch": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Beam), ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                              ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "bd_reduce": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                   ctypes.c_int),
     "bd_phase_map": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
@@ -560,6 +563,20 @@ class Context:
 
 
 _cache = {}
+
+
+def epoch_gang_launch(ctxs, beams, n_substeps, t_before, epochs, seed_bases):
+    """One epoch of several contexts with their transport in a single launch
+    (bd_epoch_gang_launch); collect each with Context.epoch_graph_finish()."""
+    n = len(ctxs)
+    if not (len(beams) == len(t_before) == len(epochs) == len(seed_bases) == n):
+        raise ValueError("epoch_gang_launch: one beam, time, epoch and seed per context")
+    hs = (ctypes.c_void_p * n)(*[c.handle for c in ctxs])
+    bs = (Beam * n)(*beams)
+    ts = (ctypes.c_double * n)(*[float(t) for t in t_before])
+    es = (ctypes.c_int64 * n)(*[int(e) for e in epochs])
+    ss = (ctypes.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in seed_bases])
+    check(lib().bd_epoch_gang_launch(hs, n, bs, int(n_substeps), ts, es, ss), "bd_epoch_gang_launch")
 
 
 def scratch_context(shape, spacing) -> Context:

@@ -9,11 +9,20 @@ cuticle-like layers mix chitin, protein, water, melanin and ommochrome, soft
 layers water, protein and lipid; reduced scattering a' U[20, 130] /cm at
 500 nm, g500 U[0.85, 0.95].  The campaign is run the way the reference's
 matrix runner sweeps wavelengths and powers (run_multispectral_matrix.py), but
-with all runs resident at once: every epoch is a captured HIP graph per run,
-and each round launches all 24 graphs (one stream per run) before waiting for
-any of them, so small runs fill the gaps of large ones.
+with all runs resident at once.  Two ways to run a round (one epoch of every
+run):
+- concurrent (default): every epoch is a captured HIP graph per run, and each
+  round launches all 24 graphs (one stream per run) before waiting for any of
+  them, so small runs fill the gaps of large ones;
+- gang: the photon transport of all 24 runs is one kernel launch
+  (bd_epoch_gang_launch), whose blocks each trace photons of one run, so the
+  long-photon tails of the strongly scattering runs fill with other runs'
+  blocks; every other step of an epoch runs on the run's own stream.  The
+  launch is a barrier between the rounds' transport and their sub-steps, which
+  costs more than the fuller transport gains on this run set (measured:
+  130.5 simulated ms/s against 144.3 concurrent, 71.4 sequential).
 
-    python -m simulation.campaign --epochs 200 [--sequential]
+    python -m simulation.campaign --epochs 200 [--mode concurrent|gang|sequential]
 """
 
 from __future__ import annotations
@@ -72,10 +81,15 @@ def campaign_runs(seed=0, n_photons=150_000, voxel_mm=0.015, plume_grid=None, gr
     return runs
 
 
-def run_campaign(runs, epochs, concurrent=True, progress=None):
-    """Run every engine for `epochs` graph-captured epochs.  Returns the wall
-    time of the epoch loop and each run's final diagnostics."""
+def run_campaign(runs, epochs, concurrent=True, progress=None, mode=None):
+    """Run every engine for `epochs` epochs, mode "gang", "concurrent" (graph
+    epochs on one stream per run) or "sequential" (one run after another; the
+    default mode follows `concurrent`).  Returns the wall time of the epoch loop,
+    each run's final diagnostics and the voxel-epochs done."""
     from .coupled_ablation import CoupledAblationEngine
+    mode = mode or ("concurrent" if concurrent else "sequential")
+    if mode not in ("gang", "concurrent", "sequential"):
+        raise ValueError(f"unknown campaign mode {mode!r}")
     engines = [CoupledAblationEngine(graph=True, **kw) for kw in runs]
     try:
         # first epoch of each run outside the timed loop: graph capture and tile autotuning
@@ -85,7 +99,12 @@ def run_campaign(runs, epochs, concurrent=True, progress=None):
             e._ctx.sync()
         t0 = time.perf_counter()
         last = [None] * len(engines)
-        if concurrent:
+        if mode == "gang":
+            for ep in range(epochs):
+                last = CoupledAblationEngine.step_epochs_gang(engines)
+                if progress and (ep + 1) % progress == 0:
+                    print(f"campaign: epoch {ep + 1}/{epochs}", flush=True)
+        elif mode == "concurrent":
             for ep in range(epochs):
                 for e in engines:
                     e.step_epoch_launch()
@@ -113,13 +132,13 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--photons", type=int, default=150_000)
-    ap.add_argument("--sequential", action="store_true", help="one run after another (for comparison)")
+    ap.add_argument("--mode", default="concurrent", choices=("concurrent", "gang", "sequential"))
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
     runs = campaign_runs(seed=a.seed, n_photons=a.photons)
-    wall, summary, vox_ep = run_campaign(runs, a.epochs, concurrent=not a.sequential, progress=25)
+    wall, summary, vox_ep = run_campaign(runs, a.epochs, progress=25, mode=a.mode)
     sim_ms = len(runs) * a.epochs * 1.0
-    print(json.dumps({"runs": len(runs), "epochs": a.epochs, "mode": "sequential" if a.sequential else "concurrent",
+    print(json.dumps({"runs": len(runs), "epochs": a.epochs, "mode": a.mode,
                       "wall_s": round(wall, 3), "simulated_ms_per_s": round(sim_ms / wall, 2),
                       "voxel_epochs_per_s": round(vox_ep / wall, 1), "summary": summary}), flush=True)
 

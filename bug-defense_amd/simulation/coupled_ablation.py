@@ -490,6 +490,32 @@ class CoupledAblationEngine:
         self._pending = (self.t + self.dt_epoch, time.perf_counter())
         self._ctx.epoch_graph_launch(beam, self.n_substeps, self.t, self.epoch, self.seed)
 
+    @staticmethod
+    def step_epochs_gang(engines):
+        """One epoch of every engine, their photon transport in a single launch
+        (bd_epoch_gang_launch); the rest of each epoch runs on the engine's own
+        stream.  The engines need a graph-capturable configuration (even
+        sub-step count, diffusion plume without blast expansion) and one
+        sub-step count.  Returns each engine's diagnostics, as step_epoch."""
+        if not engines:
+            return []
+        ns = {e.n_substeps for e in engines}
+        if len(ns) != 1:
+            raise ValueError("step_epochs_gang: the engines must share the sub-step count")
+        beams = []
+        for e in engines:
+            beams.append(e._epoch_beam())
+            if not e.graphable():
+                raise ValueError("step_epochs_gang needs graph-capturable configurations")
+        t0 = time.perf_counter()
+        B.epoch_gang_launch([e._ctx for e in engines], beams, engines[0].n_substeps, [e.t for e in engines],
+                            [e.epoch for e in engines], [e.seed for e in engines])
+        out = []
+        for e in engines:
+            d = e._ctx.epoch_graph_finish()
+            out.append(e._finish_epoch(d, e.t + e.dt_epoch, time.perf_counter() - t0))
+        return out
+
     def step_epoch_finish(self) -> dict:
         t_after, t0 = self._pending
         d = self._ctx.epoch_graph_finish()

@@ -231,6 +231,19 @@ int bd_epoch_graph_launch(bd_ctx *ctx, const bd_beam *beam, int n_substeps, doub
                           uint64_t seed_base);
 int bd_epoch_graph_finish(bd_ctx *ctx, bd_epoch_diag *out);
 
+/* One epoch of n contexts (a campaign) with all their photon transport in a
+ * single kernel launch: each context's optics pack and tally clears run on its
+ * own stream, one launch traces every context's photons (each block traces
+ * photons of one context, so the tails of long-photon runs fill with other
+ * runs' blocks), and each context's sub-steps, phase update and plume follow
+ * on its own stream.  Same per-context inputs as bd_epoch_graph_launch
+ * (arrays of n); collect each context with bd_epoch_graph_finish.  The traced
+ * photons are those of n separate epochs; only the order of the float tally
+ * additions differs.  No reference counterpart: the reference runs a campaign
+ * one run after another (run_multispectral_matrix.py). */
+int bd_epoch_gang_launch(bd_ctx *const *ctxs, int n, const bd_beam *beams, int n_substeps, const double *t_before,
+                         const int64_t *epochs, const uint64_t *seed_bases);
+
 /* Reductions (FP64): op 0 sum, 1 max, 2 min over a field; optional u8 mask
  * field (-1 for none) restricts the reduction to mask != 0. */
 int bd_reduce(bd_ctx *ctx, int field, int op, int mask_field, double *out);

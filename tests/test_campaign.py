@@ -42,3 +42,30 @@ def test_concurrent_graph_epochs_match_sequential_runs():
         # over several epochs of removal that moves the absorbed fraction at the 1e-3 level
         assert a['absorbed_frac'] == pytest.approx(b['absorbed_frac'], rel=1e-2)
         assert abs(a['removed_total'] - b['removed_total']) <= max(2, 0.01 * b['removed_total'])
+
+
+@pytest.mark.gpu
+def test_gang_transport_epochs_match_sequential_runs():
+    """All runs' photon transport in one launch traces the same photons as the
+    runs one at a time: the FP64 scalar tallies agree to round-off in the first
+    epoch, and the state after several epochs within the float-atomic spread."""
+    from simulation.campaign import run_campaign
+    from simulation.coupled_ablation import CoupledAblationEngine
+    runs = campaign_runs(n_photons=20_000, grids=((48, 48, 64), (64, 48, 72), (40, 56, 64)),
+                         wavelengths=(532.0, 1470.0), powers=(40.0,))
+    ga = [CoupledAblationEngine(graph=True, **kw) for kw in runs]
+    sq = [CoupledAblationEngine(graph=True, **kw) for kw in runs]
+    d_g = CoupledAblationEngine.step_epochs_gang(ga)
+    d_s = [e.step_epoch() for e in sq]
+    for a, b in zip(d_g, d_s):
+        assert a['absorbed_frac'] == pytest.approx(b['absorbed_frac'], rel=1e-9)
+        assert a['mc_energy_closure'] == pytest.approx(b['mc_energy_closure'], rel=1e-9)
+        for k in ('reflected', 'transmitted', 'lateral'):
+            assert a['ledger_J'][k] == pytest.approx(b['ledger_J'][k], rel=1e-9, abs=1e-15)
+    for e in ga + sq:
+        e.close()
+    w1, s1, _ = run_campaign(runs, 6, mode="gang")
+    w2, s2, _ = run_campaign(runs, 6, mode="sequential")
+    for a, b in zip(s1, s2):
+        assert a['absorbed_frac'] == pytest.approx(b['absorbed_frac'], rel=1e-2)
+        assert abs(a['removed_total'] - b['removed_total']) <= max(2, 0.01 * b['removed_total'])
