@@ -145,7 +145,7 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     if (const char *e = std::getenv("BUGDEF_MC_BLOCKS_PER_CU")) c->mc_blocks_per_cu = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("BUGDEF_SUBSTEP_PAIRS")) c->two_step_allowed = std::atoi(e) != 0;
     BD_HIP(hipHostMalloc(&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
-    BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + 1024)));
+    BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + 8192)));
     BD_TRY(ensure(c, (void **)&c->counter, sizeof(unsigned long long) * 16));
     *out = h;
     return BD_OK;

@@ -84,7 +84,7 @@ struct Ctx {
     size_t splat_tmp_bytes = 0;
 
     // scratch
-    double *red = nullptr;               // reduction accumulators (64 doubles)
+    double *red = nullptr;               // reduction accumulators (64 doubles) + 8192 doubles of partials
     double *lat = nullptr;               // lateral escape per depth
     int64_t lat_n = 0;
     unsigned long long *counter = nullptr;  // photon work counter + misc ints

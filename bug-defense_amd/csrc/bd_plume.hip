@@ -412,7 +412,7 @@ int launch_plume_update_sep(Ctx *c, double dt_epoch)
 // (coupled_ablation.py:594-603): per-block partial sum and count over ablated
 // voxels of the plume value at the voxel's plume cell, and the block's peak bits;
 // one write per block, no contended atomics
-constexpr int PD_BLOCKS = 256, PD_THREADS = 256;
+constexpr int PD_BLOCKS = 2048, PD_THREADS = 256;  // 3 x PD_BLOCKS partials fit the 8192-double scratch
 __global__ __launch_bounds__(PD_THREADS) void k_plume_diag1(const float *__restrict__ E,
                                                             const uint8_t *__restrict__ flags, int Nx, int Ny,
                                                             int Nz, int Px, int Py, int Pz, float ra,
@@ -425,15 +425,55 @@ __global__ __launch_bounds__(PD_THREADS) void k_plume_diag1(const float *__restr
     unsigned pk = 0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < NP; c += stride)
         pk = max(pk, __float_as_uint(E[c] * ra));
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += stride) {
-        if (!(flags[c] & F_ABL)) continue;
+    auto visit = [&](int64_t c) {
         const int k = (int)(c % Nz);
         const int64_t r = c / Nz;
         const int j = (int)(r % Ny), i = (int)(r / Ny);
         const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
         s += (double)(E[pc] * ra);
         cnt += 1.0;
+    };
+    // the removed-voxel mask, 16 flag bytes per load; most chunks hold none.  In
+    // a chunk that does, the 16 plume gathers are issued together (unrolled,
+    // predicated) and then summed in voxel order.
+    int64_t c0 = 0;
+    if (((uintptr_t)flags & 15) == 0) {
+        const int64_t n16 = N / 16;
+        const uint4 *f16 = (const uint4 *)flags;
+        const uint32_t b4 = (uint32_t)F_ABL * 0x01010101u;
+        for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n16; q += stride) {
+            const uint4 w = f16[q];
+            if (!((w.x | w.y | w.z | w.w) & b4)) continue;
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+            const int64_t cq = q * 16;
+            int k = (int)(cq % Nz);
+            const int64_t r = cq / Nz;
+            int j = (int)(r % Ny), i = (int)(r / Ny);
+            float v[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const bool on = (ws[e >> 2] >> (8 * (e & 3))) & F_ABL;
+                const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
+                v[e] = on ? E[pc] * ra : -1.f;
+                if (++k == Nz) {
+                    k = 0;
+                    if (++j == Ny) {
+                        j = 0;
+                        ++i;
+                    }
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                if (v[e] >= 0.f) {
+                    s += (double)v[e];
+                    cnt += 1.0;
+                }
+        }
+        c0 = n16 * 16;
     }
+    for (int64_t c = c0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += stride)
+        if (flags[c] & F_ABL) visit(c);
     __shared__ double ss[PD_THREADS], sc[PD_THREADS];
     __shared__ unsigned sp[PD_THREADS];
     ss[threadIdx.x] = s;
@@ -500,7 +540,7 @@ int launch_plume_diag_enqueue(Ctx *c)
 {
     const bd_plume_params &p = c->plume;
     double *acc = c->red + 24;
-    double *part = c->red + 64;  // the reduction partials (3 x PD_BLOCKS <= 1024 doubles)
+    double *part = c->red + 64;  // the reduction partials (3 x PD_BLOCKS <= 8192 doubles)
     double dx, dy, dz;
     plume_spacing(c, dx, dy, dz);
     const int I = (int)(((int64_t)(2 * (c->g.nx / 2) + 1) * p.nx) / (2 * (int64_t)c->g.nx));

@@ -114,7 +114,35 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const void *__re
 {
     __shared__ double s[RED_THREADS];
     double acc = op == 0 ? 0.0 : (op == 1 ? -1.0e308 : 1.0e308);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i0 = 0;
+    if constexpr (KIND == K_F32) if (((uintptr_t)src & 15) == 0 && ((uintptr_t)mask & 3) == 0) {
+        // four elements per load (float4, and the four mask bytes as one word)
+        const int64_t n4 = n / 4;
+        const float4 *s4 = (const float4 *)src;
+        const uint32_t *m4 = (const uint32_t *)mask;
+        const uint32_t b4 = (uint32_t)bits * 0x01010101u;
+        for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n4; q += (int64_t)gridDim.x * blockDim.x) {
+            uint32_t keep = 0xffffffffu;
+            if (mask) {
+                const uint32_t hit = m4[q] & b4;  // per byte: nonzero where the bit is set
+                keep = 0;
+                for (int e = 0; e < 4; ++e) {
+                    const bool m = ((hit >> (8 * e)) & 0xffu) != 0;
+                    if (m != (invert != 0)) keep |= 0xffu << (8 * e);
+                }
+                if (!keep) continue;
+            }
+            const float4 v4 = s4[q];
+            const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+            for (int e = 0; e < 4; ++e) {
+                if (!((keep >> (8 * e)) & 1u)) continue;
+                const double v = vv[e];
+                acc = op == 0 ? acc + v : (op == 1 ? fmax(acc, v) : fmin(acc, v));
+            }
+        }
+        i0 = n4 * 4;
+    }
+    for (int64_t i = i0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (mask) {
             const bool m = (mask[i] & bits) != 0;
             if (m == (invert != 0)) continue;
