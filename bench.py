@@ -98,17 +98,22 @@ def run_plain(kw, steps, warmup, graph):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_substep_traffic_layered_256x256x192.md")
 
 
-def measured_traffic():
+def measured_traffic(tuning):
     """HBM bytes per one-step sub-step launch on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None when
+    that summary was measured with another tile shape than this run's."""
+    tile, traffic = None, None
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("- traffic_bytes_per_launch:"):
-                    return int(float(line.split(":")[1]))
-    except OSError:
-        pass
-    return None
+                if line.startswith("- tile:"):
+                    by, xc = line.split(":")[1].split("(")[0].split("x")
+                    tile = (int(by), int(xc))
+                elif line.startswith("- traffic_bytes_per_launch:"):
+                    traffic = int(float(line.split(":")[1]))
+    except (OSError, ValueError):
+        return None
+    return traffic if tile == (tuning["substep_tile_y"], tuning["substep_tile_x"]) else None
 
 
 def run_surrogate(kw, steps, warmup, weights):
@@ -252,7 +257,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None if n_pair else measured_traffic(),
+            "traffic": None if n_pair else measured_traffic(tun),
             "bytes_per_voxel": SUBSTEP_BYTES_PER_VOXEL,
             "avg_launch_ms": round(sub_launch_ms, 5),
             "launches_per_epoch": n_launch,

@@ -2,6 +2,8 @@
 // field transfer and the per-call orchestration of the kernels.
 #include "bd_internal.hpp"
 
+#include <cstdio>
+
 #include <cstdlib>
 
 #include <cmath>
@@ -144,6 +146,16 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     for (int i = 0; i < 6; ++i) BD_HIP(hipEventCreate(&c->gev[i]));
     if (const char *e = std::getenv("BUGDEF_MC_BLOCKS_PER_CU")) c->mc_blocks_per_cu = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("BUGDEF_SUBSTEP_PAIRS")) c->two_step_allowed = std::atoi(e) != 0;
+    // a fixed one-step tile "BYxXC" (BY 2, 4 or 8; XC >= 1) in place of the timed choice, e.g. for
+    // counter runs, where the profiler's per-dispatch overhead distorts the tuning launches
+    if (const char *e = std::getenv("BUGDEF_SUBSTEP_TILE")) {
+        int by = 0, xc = 0;
+        if (std::sscanf(e, "%dx%d", &by, &xc) == 2 && (by == 2 || by == 4 || by == 8) && xc >= 1) {
+            c->tile_by = by;
+            c->tile_xc = xc;
+            c->tile_pinned = true;
+        }
+    }
     BD_HIP(hipHostMalloc(&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
     BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + 8192)));
     BD_TRY(ensure(c, (void **)&c->counter, sizeof(unsigned long long) * 16));

@@ -103,6 +103,7 @@ struct Ctx {
     float *Hb = nullptr;               // second H_acc buffer (the two-sub-step kernel writes the other one)
     uint8_t *flagsb = nullptr;         // second flags buffer
     bool tuned = false;
+    bool tile_pinned = false;  // one-step tile fixed by BUGDEF_SUBSTEP_TILE: no timing pass
     double last_ledger_conducted = 0.0;
 
     // captured epoch graph (bd_epoch_graph) and the beam it was captured for

@@ -568,6 +568,10 @@ static int autotune(Ctx *c)
 {
     if (c->tuned) return BD_OK;
     c->tuned = true;
+    if (c->tile_pinned) {  // fixed one-step tile, no pairs
+        c->use_two = false;
+        return BD_OK;
+    }
     if (c->nvox < (1 << 20)) {  // small grids: fixed shapes, nothing to win
         c->tile_by = 4;
         c->tile_xc = 8;
@@ -580,50 +584,69 @@ static int autotune(Ctx *c)
         }
         return BD_OK;
     }
-    // staging: T copy [0, 4n), H copy [4n, 8n), flags scratch [8n, 9n); sized once, before
-    // any pointer into it is taken
-    BD_TRY(ensure_staging(c, (size_t)c->nvox * 10));
-    // scratch copies so the tuning launches leave the state untouched
+    // Staging: T [0, 4n), H [4n, 8n), chi [8n, 12n), flags [12n, 13n), and the
+    // pair kernel's T output [13n, 17n) -- scratch copies, so the tuning launches
+    // run the epoch's per-cell work (plateau, char) and leave the state
+    // untouched.  Sized once, before any pointer into it is taken.
+    const size_t n = (size_t)c->nvox;
+    BD_TRY(ensure_staging(c, n * 17));
     float *Tscr = (float *)c->staging;
-    float *Hscr = (float *)((char *)c->staging + (size_t)c->nvox * 4);
-    BD_HIP(hipMemcpyAsync(Tscr, c->T, (size_t)c->nvox * 4, hipMemcpyDeviceToDevice, c->stream));
-    BD_HIP(hipMemcpyAsync(Hscr, c->H, (size_t)c->nvox * 4, hipMemcpyDeviceToDevice, c->stream));
+    float *Hscr = (float *)((char *)c->staging + n * 4);
+    float *Cscr = (float *)((char *)c->staging + n * 8);
+    uint8_t *Fscr = (uint8_t *)((char *)c->staging + n * 12);
+    const bool chr = c->chi && c->phys.has_char;
+    BD_HIP(hipMemcpyAsync(Tscr, c->T, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    BD_HIP(hipMemcpyAsync(Hscr, c->H, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    BD_HIP(hipMemcpyAsync(Fscr, c->flags, n, hipMemcpyDeviceToDevice, c->stream));
+    if (chr) BD_HIP(hipMemcpyAsync(Cscr, c->chi, n * 4, hipMemcpyDeviceToDevice, c->stream));
     const int bys[] = {2, 4, 8};
     const int xcs[] = {8, 16, 32, 64};
+    float tms[3][4];
     float best = 1e30f;
-    int bb = 4, bx = 16;
-    for (int by : bys)
-        for (int xc : xcs) {
-            c->tile_xc = xc;
+    for (int ib = 0; ib < 3; ++ib)
+        for (int ix = 0; ix < 4; ++ix) {
+            c->tile_xc = xcs[ix];
             SubArgs a{};
-            fill_args(c, a, 1, BD_SUB_PLATEAU);
+            fill_args(c, a, 1, BD_SUB_PLATEAU | BD_SUB_CHAR | BD_SUB_MASK_Q);
             a.Tin = Tscr;
             a.Tout = c->T2;
             a.H = Hscr;
-            a.flags = nullptr;  // never written: plateau reads flags; give a private copy below
-            a.flags = c->flags;
-            a.plateau = 0;      // stencil + loads only; the plateau writes would perturb the state
-            a.chr = 0;
+            a.flags = Fscr;
+            a.chi = Cscr;
+            a.chr = chr ? 1 : 0;
             a.arr = 0;
             float ms = 0.f;
-            BD_TRY(launch_one(c, a, by));  // warm
+            BD_TRY(launch_one(c, a, bys[ib]));  // warm
             BD_HIP(hipEventRecord(c->ev0, c->stream));
-            for (int r = 0; r < 3; ++r) BD_TRY(launch_one(c, a, by));
+            for (int r = 0; r < 3; ++r) BD_TRY(launch_one(c, a, bys[ib]));
             BD_HIP(hipEventRecord(c->ev1, c->stream));
             BD_HIP(hipEventSynchronize(c->ev1));
             BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            if (ms < best) {
-                best = ms;
-                bb = by;
-                bx = xc;
-            }
+            tms[ib][ix] = ms;
+            best = std::min(best, ms);
         }
+    // among shapes within 3 % of the fastest, the tallest tile (fewest y-halo
+    // re-reads of T and k), then the longest x march: timing noise does not
+    // flip the choice between runs
+    int bb = 4, bx = 16;
+    for (int ib = 2; ib >= 0; --ib) {
+        int got = -1;
+        for (int ix = 3; ix >= 0; --ix)
+            if (tms[ib][ix] <= 1.03f * best) {
+                got = ix;
+                break;
+            }
+        if (got >= 0) {
+            bb = bys[ib];
+            bx = xcs[got];
+            break;
+        }
+    }
     c->tile_by = bb;
     c->tile_xc = bx;
     // two-sub-step kernel: tile shape, and whether it beats two one-step launches
-    uint8_t *Fscr = (uint8_t *)((char *)c->staging + (size_t)c->nvox * 8);
     float *Hscr2 = (float *)c->T2;  // output H of the tuning launches (T2 is scratch between sub-steps)
-    BD_HIP(hipMemsetAsync(Fscr, 0, (size_t)c->nvox, c->stream));
+    BD_HIP(hipMemcpyAsync(Fscr, c->flags, n, hipMemcpyDeviceToDevice, c->stream));
     const int bys2[] = {8, 16};
     const int xcs2[] = {16, 32, 64};
     float best2 = 1e30f;
@@ -631,15 +654,15 @@ static int autotune(Ctx *c)
     for (int by : bys2)
         for (int xc : xcs2) {
             SubArgs a{};
-            fill_args(c, a, 1, BD_SUB_PLATEAU);
+            fill_args(c, a, 1, BD_SUB_PLATEAU | BD_SUB_CHAR | BD_SUB_MASK_Q);
             a.Tin = Tscr;
-            a.Tout = (float *)((char *)c->staging + (size_t)c->nvox * 4);  // Hscr slot reused as T output
+            a.Tout = (float *)((char *)c->staging + n * 13);
             a.H_in = Hscr;
             a.H = Hscr2;
             a.flags_in = c->flags;
             a.flags = Fscr;
-            a.plateau = 0;
-            a.chr = 0;
+            a.chi = Cscr;
+            a.chr = chr ? 1 : 0;
             a.arr = 0;
             float ms = 0.f;
             BD_TRY(launch_two(c, a, by, xc));
