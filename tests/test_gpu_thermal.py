@@ -245,3 +245,38 @@ def test_paired_substeps_bit_identical_to_single_substeps(shape, monkeypatch):
         c.close()
     for n in out[0]:
         np.testing.assert_array_equal(out[0][n], out[1][n], err_msg=n)
+
+
+def test_substep_tile_shape_does_not_change_the_result(monkeypatch):
+    """Every tile shape evaluates each cell from the same inputs in the same order,
+    so the fields after 20 sub-steps are bit-identical across the shapes the
+    timing pass can pick (fixed here with BUGDEF_SUBSTEP_TILE)."""
+    from simulation import _bdlib as B
+    shape = (70, 45, 130)   # partial tiles in every axis
+    rng = np.random.default_rng(12)
+    dx, dt = 0.015, 5e-5
+    k = rng.uniform(0.1e-3, 0.6e-3, shape)
+    rc = rng.uniform(2.0e-3, 4.2e-3, shape)
+    T0 = rng.uniform(60.0, 140.0, shape)
+    Q = rng.uniform(0.0, 400.0, shape)
+    dry = (rng.random(shape) < 0.2).astype(np.uint8)
+    H0 = np.where(rng.random(shape) < 0.3, rng.uniform(0, 0.3, shape), 0.0)
+    chi0 = rng.uniform(0.0, 0.5, shape)
+    monkeypatch.setenv("BUGDEF_SUBSTEP_PAIRS", "0")
+    out = []
+    for tile in ("2x8", "4x16", "8x64", "4x5"):
+        monkeypatch.setenv("BUGDEF_SUBSTEP_TILE", tile)
+        c = _ctx(shape, (dx, dx, dx), dt=dt, h_eff=1.5e-5, t_ambient=20.0, t_abl=100.0, t_void=110.0,
+                 has_char=1, char_A=1.0e7, char_Ea_over_R=65000.0 / 8.314)
+        for name, a in (('T', T0), ('Q', Q), ('K', k), ('RHO_CP', rc), ('DRY', dry), ('H_ACC', H0), ('CHI', chi0)):
+            c.upload(name, a)
+        c.fill('T_PEAK', 0.0)
+        c.substeps(20, 1, B.SUB_PLATEAU | B.SUB_MASK_Q | B.SUB_CHAR | B.SUB_PEAK)
+        by, xc = (int(v) for v in tile.split("x"))
+        t = c.tuning()
+        assert (t["substep_tile_y"], t["substep_tile_x"]) == (by, xc)
+        out.append({n: c.download(n) for n in ('T', 'H_ACC', 'CHI', 'T_PEAK', 'DRY')})
+        c.close()
+    for o in out[1:]:
+        for n in out[0]:
+            np.testing.assert_array_equal(out[0][n], o[n], err_msg=n)
