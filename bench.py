@@ -139,6 +139,7 @@ def run_surrogate(kw, steps, warmup, weights):
     out = {"wall": wall, "surrogate_epochs": sum(1 for h in timed if h["optics_source"] == "surrogate"),
            "mc_epochs": sum(1 for h in timed if h["optics_source"] == "mc"),
            "judged_block4": [round(e[1], 4) for e in judged], "judged_voxel": [round(e[0], 4) for e in judged],
+           "scale_after_judges": [round(h["scale"], 4) for h in hist if h["judge_error"] is not None],
            "state": final_state(eng, 21)}
     sur.close()
     eng.close()
@@ -270,6 +271,7 @@ def main():
         result["surrogate"] = {"value": round(rate, 3), "speedup_vs_plain": round(rate / plain_rate, 3),
                                "surrogate_epochs": s["surrogate_epochs"], "mc_epochs": s["mc_epochs"],
                                "judged_rel_l2_block4": jb, "judged_rel_l2_voxel": s["judged_voxel"],
+                               "mc_anchored_scale": s["scale_after_judges"],
                                "max_judged_rel_l2_block4": max(jb) if jb else None, **s["state"]}
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -104,6 +104,8 @@ struct Ctx {
     uint8_t *flagsb = nullptr;         // second flags buffer
     bool tuned = false;
     bool tile_pinned = false;  // one-step tile fixed by BUGDEF_SUBSTEP_TILE: no timing pass
+    int64_t sub_launched = 0;  // sub-steps launched so far (eager calls)
+    bool retuned = false;      // the second timing pass, on a developed state, has run
     double last_ledger_conducted = 0.0;
 
     // captured epoch graph (bd_epoch_graph) and the beam it was captured for

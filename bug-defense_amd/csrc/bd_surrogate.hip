@@ -316,13 +316,14 @@ __global__ void k_copy_f32(const float *__restrict__ s, float *__restrict__ d, i
 }
 
 // judge error: per-block partials of sum (a - b)^2, sum a^2 over non-ablated
-// voxels, and the same over 4^3 block sums (MC shot noise averaged down)
+// voxels, the same over 4^3 block sums (MC shot noise averaged down), and the
+// totals sum a, sum b
 __global__ __launch_bounds__(256) void k_sur_err(const float *__restrict__ a, const float *__restrict__ b,
                                                  const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz,
                                                  double *__restrict__ part)
 {
-    __shared__ double s[4][256];
-    double d2 = 0.0, a2 = 0.0, bd2 = 0.0, ba2 = 0.0;
+    __shared__ double s[6][256];
+    double d2 = 0.0, a2 = 0.0, bd2 = 0.0, ba2 = 0.0, ta = 0.0, tb = 0.0;
     const int bx = (Nx + 3) / 4, by = (Ny + 3) / 4, bz = (Nz + 3) / 4;
     const int64_t nb = (int64_t)bx * by * bz;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nb; t += (int64_t)gridDim.x * blockDim.x) {
@@ -341,28 +342,32 @@ __global__ __launch_bounds__(256) void k_sur_err(const float *__restrict__ a, co
                 }
         bd2 += (sa - sb) * (sa - sb);
         ba2 += sa * sa;
+        ta += sa;
+        tb += sb;
     }
     s[0][threadIdx.x] = d2;
     s[1][threadIdx.x] = a2;
     s[2][threadIdx.x] = bd2;
     s[3][threadIdx.x] = ba2;
+    s[4][threadIdx.x] = ta;
+    s[5][threadIdx.x] = tb;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h)
-            for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + h];
+            for (int q = 0; q < 6; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + h];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; ++q) part[4 * blockIdx.x + q] = s[q][0];
+        for (int q = 0; q < 6; ++q) part[6 * blockIdx.x + q] = s[q][0];
 }
 
 __global__ void k_sur_err_final(const double *__restrict__ part, int nb, double *__restrict__ out)
 {
     if (threadIdx.x || blockIdx.x) return;
-    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int b = 0; b < nb; ++b)
-        for (int q = 0; q < 4; ++q) r[q] += part[4 * b + q];
-    for (int q = 0; q < 4; ++q) out[q] = r[q];
+        for (int q = 0; q < 6; ++q) r[q] += part[6 * b + q];
+    for (int q = 0; q < 6; ++q) out[q] = r[q];
 }
 
 namespace {
@@ -533,8 +538,10 @@ int bd_sur_targets(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float 
     return BD_OK;
 }
 
-int bd_sur_apply_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins, const float *y_dev, int into_q)
+int bd_sur_apply_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins, const float *y_dev, int into_q,
+                   double scale)
 {
+    BD_CHECK(scale > 0.0 && std::isfinite(scale), "bd_sur_apply: scale must be positive and finite");
     BD_CHECK(h && beam && origins && y_dev, "bd_sur_apply: null argument");
     BD_CHECK(n >= 1 && n <= MAXW, "bd_sur_apply: window count must be in [1, 4]");
     Ctx *c = &h->c;
@@ -575,7 +582,7 @@ int bd_sur_apply_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins
     a.y = y_dev;
     a.Qs = c->Qs;
     a.Qo = dst;
-    a.E0 = (float)(beam->power / (M_PI * (double)f.w0 * f.w0 / 2.0));
+    a.E0 = (float)(scale * beam->power / (M_PI * (double)f.w0 * f.w0 / 2.0));
     hipLaunchKernelGGL(k_sur_apply, dim3(2048), dim3(256), 0, c->stream, a);
     BD_HIP(hipGetLastError());
     return BD_OK;
@@ -583,7 +590,7 @@ int bd_sur_apply_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins
 
 int bd_sur_apply(bd_ctx *h, const bd_beam *beam, const int32_t *origin, const float *y_dev, int into_q)
 {
-    return bd_sur_apply_n(h, beam, 1, origin, y_dev, into_q);
+    return bd_sur_apply_n(h, beam, 1, origin, y_dev, into_q, 1.0);
 }
 
 int bd_sur_keep_stale(bd_ctx *h)
@@ -598,11 +605,11 @@ int bd_sur_keep_stale(bd_ctx *h)
     return BD_OK;
 }
 
-int bd_sur_error(bd_ctx *h, double *out)
+int bd_sur_judge(bd_ctx *h, double *out)
 {
-    BD_CHECK(h && out, "bd_sur_error: null argument");
+    BD_CHECK(h && out, "bd_sur_judge: null argument");
     Ctx *c = &h->c;
-    BD_CHECK(c->Qsur && c->Q, "bd_sur_error: no surrogate field to compare (bd_sur_apply with into_q = 0)");
+    BD_CHECK(c->Qsur && c->Q, "bd_sur_judge: no surrogate field to compare (bd_sur_apply with into_q = 0)");
     constexpr int NB = 256;
     double *part = c->red + 64;
     hipLaunchKernelGGL(k_sur_err, dim3(NB), dim3(256), 0, c->stream, c->Q, c->Qsur, c->flags, c->g.nx, c->g.ny,
@@ -610,11 +617,23 @@ int bd_sur_error(bd_ctx *h, double *out)
     BD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sur_err_final, dim3(1), dim3(64), 0, c->stream, part, NB, c->red + 48);
     BD_HIP(hipGetLastError());
-    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 48, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 48, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;
     out[0] = r[1] > 0.0 ? std::sqrt(r[0] / r[1]) : 0.0;
     out[1] = r[3] > 0.0 ? std::sqrt(r[2] / r[3]) : 0.0;
+    out[2] = r[4];
+    out[3] = r[5];
+    return BD_OK;
+}
+
+int bd_sur_error(bd_ctx *h, double *out)
+{
+    BD_CHECK(h && out, "bd_sur_error: null argument");
+    double r[4];
+    BD_TRY(bd_sur_judge(h, r));
+    out[0] = r[0];
+    out[1] = r[1];
     return BD_OK;
 }
 

@@ -690,7 +690,20 @@ static int autotune(Ctx *c)
 
 int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger)
 {
+    // The first timing pass runs on the initial state, where the plateau and
+    // char branches hardly fire; after 80 sub-steps (four epochs of 20) the pass
+    // runs once more on the developed state, whose per-cell work favours other
+    // shapes.  Never inside a graph capture (the pass waits on the host).
+    if (c->tuned && !c->retuned && !c->tile_pinned && c->sub_launched >= 80) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        BD_HIP(hipStreamIsCapturing(c->stream, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            c->tuned = false;
+            c->retuned = true;
+        }
+    }
     BD_TRY(autotune(c));
+    c->sub_launched += n;
     if (flags & BD_SUB_PEAK) BD_TRY(ensure(c, (void **)&c->Tpeak, (size_t)c->nvox * 4));
     if (flags & BD_SUB_ARRHENIUS) BD_TRY(ensure(c, (void **)&c->Om, (size_t)c->nvox * 4));
     SubArgs a{};

@@ -160,7 +160,8 @@ _SIGS = {
     "bd_sur_features_n": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                            ctypes.c_void_p], ctypes.c_int),
     "bd_sur_apply_n": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
-                        ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+                        ctypes.c_void_p, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "bd_sur_judge": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bd_sur_keep_stale": ([ctypes.c_void_p], ctypes.c_int),
     "bd_sur_error": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     # surrogate network (bugdef_fno.h); tensors are device pointers
@@ -512,13 +513,14 @@ class Context:
             raise ValueError("sur_features_n: x_dev holds fewer windows than origins")
         check(lib().bd_sur_features_n(self.handle, ctypes.byref(beam), n, o, x_dev.ptr), "bd_sur_features_n")
 
-    def sur_apply_n(self, beam: Beam, origins, y_dev, into_q):
-        """Compose Q from predictions y_dev [n, 64, 64, 64] of len(origins) windows."""
+    def sur_apply_n(self, beam: Beam, origins, y_dev, into_q, scale=1.0):
+        """Compose Q from predictions y_dev [n, 64, 64, 64] of len(origins) windows,
+        the network's absorbed power multiplied by `scale`."""
         o, n = self._origins(origins)
         if y_dev.shape[0] < n:
             raise ValueError("sur_apply_n: y_dev holds fewer windows than origins")
-        check(lib().bd_sur_apply_n(self.handle, ctypes.byref(beam), n, o, y_dev.ptr, 1 if into_q else 0),
-              "bd_sur_apply_n")
+        check(lib().bd_sur_apply_n(self.handle, ctypes.byref(beam), n, o, y_dev.ptr, 1 if into_q else 0,
+                                   float(scale)), "bd_sur_apply_n")
 
     def sur_keep_stale(self):
         check(lib().bd_sur_keep_stale(self.handle), "bd_sur_keep_stale")
@@ -527,6 +529,12 @@ class Context:
         out = (ctypes.c_double * 2)()
         check(lib().bd_sur_error(self.handle, out), "bd_sur_error")
         return float(out[0]), float(out[1])
+
+    def sur_judge(self):
+        """(per-voxel rel L2, block-4 rel L2, sum of Q, sum of the composed field)."""
+        out = (ctypes.c_double * 4)()
+        check(lib().bd_sur_judge(self.handle, out), "bd_sur_judge")
+        return tuple(float(v) for v in out)
 
     def reduce(self, name, op="sum", mask=None):
         ops = {"sum": 0, "max": 1, "min": 2}

@@ -16,6 +16,12 @@ judge epoch (still every ``judge_every``-th) finds the error back within
 The judged error is the relative L2 over 4x4x4 block sums of the absorbed
 power (non-ablated voxels), which averages down the MC's own shot noise; the
 per-voxel figure is recorded beside it.
+
+MC-anchored scale (``calibrate``, on by default): each judge epoch also
+compares the total absorbed power of the MC field with that of the composed
+prediction, and the following predictions are multiplied by the ratio (kept
+within [0.5, 2]).  A judged error is always that of a prediction made before
+its own epoch's MC, with the scale of the previous judge epoch.
 """
 
 import time
@@ -26,7 +32,8 @@ from .network import CIN, T, FourierSurrogate
 
 
 class SurrogateCoupledEngine:
-    def __init__(self, engine, weights, judge_every=10, tol=0.05, margin=12, metric="block4", max_windows=4):
+    def __init__(self, engine, weights, judge_every=10, tol=0.05, margin=12, metric="block4", max_windows=4,
+                 calibrate=True):
         if engine.plume is not None and engine.plume.model != 'diffusion':
             raise ValueError("the surrogate loop supports the diffusion plume only")
         self.engine = engine
@@ -42,6 +49,8 @@ class SurrogateCoupledEngine:
         self.x = DeviceArray((self.max_windows, T, T, T, CIN))
         self.y = DeviceArray((self.max_windows, T, T, T))
         self.windows = []
+        self.calibrate = bool(calibrate)
+        self.scale = 1.0
         self.fallback = False
         self.have_stale = False
         self.judged = False
@@ -54,7 +63,7 @@ class SurrogateCoupledEngine:
         ctx.sur_features_n(beam, ws, self.x)
         ctx.sync()
         self.net.forward(self.x.view(0, (nb, T, T, T, CIN)), self.y.view(0, (nb, T, T, T)))
-        ctx.sur_apply_n(beam, ws, self.y, into_q)
+        ctx.sur_apply_n(beam, ws, self.y, into_q, self.scale)
         self.windows = ws
         return ws
 
@@ -79,10 +88,13 @@ class SurrogateCoupledEngine:
                 self._predict(beam, into_q=False)
             d = eng.step_epoch()
             if compare:
-                err = eng._ctx.sur_error()
+                j = eng._ctx.sur_judge()
+                err = (j[0], j[1])
                 e = err[1] if self.metric == "block4" else err[0]
                 self.fallback = e > self.tol
                 self.judged = True
+                if self.calibrate and j[2] > 0.0 and j[3] > 0.0:
+                    self.scale = min(2.0, max(0.5, self.scale * j[2] / j[3]))
             eng._ctx.sur_keep_stale()
             self.have_stale = True
             d['optics_source'] = 'mc'
@@ -96,7 +108,9 @@ class SurrogateCoupledEngine:
         d['t_wall_s'] = time.perf_counter() - t0
         d['fallback'] = self.fallback
         d['windows'] = len(self.windows)
-        self.history.append({k: d[k] for k in ('epoch', 'optics_source', 'judge_error', 'fallback', 'n_new', 'windows')})
+        d['scale'] = self.scale
+        self.history.append({k: d[k] for k in ('epoch', 'optics_source', 'judge_error', 'fallback', 'n_new', 'windows',
+                                               'scale')})
         return d
 
     def close(self):

@@ -120,9 +120,11 @@ int bd_sur_apply(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, const 
  * mean of the windows holding it, sum_k r_k q_k / max(W, 1), plus
  * (1 - min(W, 1)) of the kept field, W = sum_k r_k (r_k: the 8-voxel face
  * ramps of window k at its faces inside the domain; overlapping ramps sum
- * to one). */
+ * to one).  The network's absorbed power is multiplied by `scale` (> 0): the
+ * coupled loop passes the MC / network ratio of the total absorbed power from
+ * its last judge epoch. */
 int bd_sur_apply_n(bd_ctx *ctx, const bd_beam *beam, int n, const int32_t *origins, const float *y_dev,
-                   int into_q);
+                   int into_q, double scale);
 
 /* Keep the current Q (an MC epoch's field) as the field outside the window. */
 int bd_sur_keep_stale(bd_ctx *ctx);
@@ -130,6 +132,11 @@ int bd_sur_keep_stale(bd_ctx *ctx);
 /* Relative L2 error of the composed field against Q over non-ablated voxels:
  * out[0] per voxel, out[1] over 4x4x4 block sums (MC shot noise averaged). */
 int bd_sur_error(bd_ctx *ctx, double *out);
+
+/* The same comparison with the totals: out[0] per-voxel and out[1] block-4
+ * relative L2 as bd_sur_error, out[2] the sum of Q and out[3] the sum of the
+ * composed field, both over non-ablated voxels. */
+int bd_sur_judge(bd_ctx *ctx, double *out);
 
 /* Plain device buffers for the tensors above, so a host program needs no other
  * GPU runtime: allocation on `device`, synchronous host<->device and

@@ -210,12 +210,13 @@ def _ramp(lo, hi):
     return r
 
 
-def apply_n(ys, mu_a, Qs, origins, power, spot):
+def apply_n(ys, mu_a, Qs, origins, power, spot, scale=1.0):
     """Composed absorbed-power field over several windows (float64): per voxel
-    sum_k r_k q_k / max(W, 1) + (1 - min(W, 1)) Qs with W = sum_k r_k."""
+    sum_k r_k q_k / max(W, 1) + (1 - min(W, 1)) Qs with W = sum_k r_k, the
+    network's q_k multiplied by `scale`."""
     Nx, Ny, Nz = mu_a.shape
     w0 = spot / 2.0
-    E0 = power / (math.pi * w0 * w0 / 2.0)
+    E0 = scale * power / (math.pi * w0 * w0 / 2.0)
     n = (Nx, Ny, Nz)
     S = np.zeros(mu_a.shape)
     W = np.zeros(mu_a.shape)

@@ -148,6 +148,16 @@ def test_tiled_windows_features_and_apply_match_oracle(phi_deg):
     c.sur_apply_n(beam, ws, yd, into_q=True)
     want = S.apply_n([_f32(y) for y in ys], _f32(props['mu_a']), _f32(Qs), wo, 40.0, 0.5)
     np.testing.assert_allclose(c.download('Q'), want, rtol=2e-6, atol=1e-6)
+    # the scaled composition, and the judge's totals against the composed field
+    c.sur_apply_n(beam, ws, yd, into_q=False, scale=1.7)
+    want = S.apply_n([_f32(y) for y in ys], _f32(props['mu_a']), _f32(Qs), wo, 40.0, 0.5, scale=1.7)
+    j = c.sur_judge()
+    q = c.download('Q')   # the unscaled composition written above
+    keep = ~abl
+    assert j[2] == pytest.approx(float(q[keep].sum()), rel=1e-6)
+    assert j[3] == pytest.approx(float(want[keep].sum()), rel=1e-5)
+    ew = S.rel_error(q, want, abl)
+    assert j[0] == pytest.approx(ew[0], rel=1e-5) and j[1] == pytest.approx(ew[1], rel=1e-5)
     c.close()
 
 

@@ -127,6 +127,10 @@ def test_apply_n_is_a_partition_of_unity_in_the_overlap():
     inner = (slice(16, 64), slice(11, 59), slice(32, 136))   # away from the outer faces
     np.testing.assert_allclose(out[inner], F[inner], rtol=1e-10)
     assert out[0, 0, 0] == Qs[0, 0, 0] and out[79, 69, 149] == Qs[79, 69, 149]
+    # the scale multiplies the network's part only (F >> 1e-4 E0 mu_a, so the offset is negligible)
+    out2 = S.apply_n(ys, ma, Qs, ws, 40.0, 0.5, scale=1.5)
+    np.testing.assert_allclose(out2[inner], 1.5 * F[inner], rtol=1e-3)
+    assert out2[0, 0, 0] == Qs[0, 0, 0]
     # one window reduces to apply()
     np.testing.assert_allclose(S.apply_n(ys[:1], ma, Qs, ws[:1], 40.0, 0.5), S.apply(ys[0], ma, Qs, ws[0], 40.0, 0.5),
                                rtol=1e-12)
