@@ -72,7 +72,7 @@ def run_plain(kw, steps, warmup, graph):
     log(f"bench: plain loop, {warmup} warmup epochs done")
     eng._ctx.sync()
     rec = {k: [] for k in ("sub", "mc", "opt", "th", "ph")}
-    events, nnew = 0.0, 0
+    events, atomics, nnew = 0.0, 0.0, 0
     t0 = time.perf_counter()
     for i in range(steps):
         d = eng.step_epoch()
@@ -83,12 +83,13 @@ def run_plain(kw, steps, warmup, graph):
         rec["th"].append(ld.t_thermal_ms)
         rec["ph"].append(ld.t_phase_ms)
         events += ld.mc_events
+        atomics += ld.mc_tally_atomics
         nnew += d['n_new']
         if (i + 1) % 25 == 0:
             log(f"bench: plain epoch {i + 1}/{steps}")
     eng._ctx.sync()
     wall = time.perf_counter() - t0
-    out = {"wall": wall, "events": events, "nnew": nnew, "nvox": eng.Nx * eng.Ny * eng.Nz, "tuning": eng._ctx.tuning(),
+    out = {"wall": wall, "events": events, "atomics": atomics, "nnew": nnew, "nvox": eng.Nx * eng.Ny * eng.Nz, "tuning": eng._ctx.tuning(),
            "n_sub": eng.n_substeps, "dt_epoch": eng.dt_epoch, "n_photons": eng.n_photons,
            **{k: np.asarray(v) for k, v in rec.items()}, "state": final_state(eng, 21)}
     eng.close()
@@ -160,7 +161,8 @@ def main():
                     help="only run three 256 MiB device-to-device copies (calibrates the HBM byte counters)")
     ap.add_argument("--mc-probe", default=None, metavar="N1,N2,...",
                     help="only time the transport kernel on the workload's state after three epochs, at these "
-                         "photon counts (three launches each; for profiling the MC kernel)")
+                         "photon counts (three launches each, with its tally atomics), then the atomic-rate probe "
+                         "over a range of address spans (for profiling the MC kernel)")
     args = ap.parse_args()
     if args.mc_probe:
         from simulation.coupled_ablation import CoupledAblationEngine
@@ -172,15 +174,22 @@ def main():
         out = {}
         for n in (int(v) for v in args.mc_probe.split(",")):
             beam.n_photons = n
-            ts = []
+            ts, atom = [], 0.0
             for rep in range(3):
                 beam.seed = 77 + rep
-                ts.append(eng._ctx.mc_run(beam).kernel_ms)
+                r = eng._ctx.mc_run(beam)
+                ts.append(r.kernel_ms)
+                atom = r.tally_atomics
             out[n] = {"kernel_ms_min": round(min(ts), 3), "kernel_ms_max": round(max(ts), 3),
-                      "photons_per_s": round(n / (min(ts) * 1e-3), 1)}
+                      "photons_per_s": round(n / (min(ts) * 1e-3), 1), "tally_atomics": int(atom),
+                      "g_atomics_per_s": round(atom / (min(ts) * 1e-3) / 1e9, 2)}
             log(f"bench: mc probe {n} photons done")
+        nvox = eng.Nx * eng.Ny * eng.Nz
         eng.close()
-        print(json.dumps({"mc_probe": out}))
+        from simulation import _bdlib as B
+        spans = {str(sp): round(B.probe_atomic_rate(nvox, sp) / 1e9, 2)
+                 for sp in (nvox, 262144, 65536, 32768, 8192, 4096, 2048)}
+        print(json.dumps({"mc_probe": out, "atomic_probe_g_per_s_by_span": spans}))
         return
     if args.copy_kernel:
         from simulation import _bdlib as B
@@ -214,6 +223,12 @@ def main():
     achieved = SUBSTEP_BYTES_PER_VOXEL * p["nvox"] / (sub_launch_ms * 1e-3) / 1e9
     per_substep_ms = float(np.mean(p["sub"])) / p["n_sub"]
     mc_avg = float(np.mean(p["mc"]))
+    # the transport's tally atomics against the device's measured atomic-add rate on
+    # the same float2 grid: pairs hashed over the whole grid, and over 64k voxels
+    probe = {f"uniform_{name}": round(B.probe_atomic_rate(p["nvox"], span) / 1e9, 2)
+             for name, span in (("full_grid", p["nvox"]), ("65536_voxels", 65536))}
+    mc_atomic_rate = p["atomics"] / (float(np.sum(p["mc"])) * 1e-3) / 1e9
+    log("bench: atomic-rate probe done")
     s = None
     if not args.plain_only:
         if not os.path.exists(args.weights):
@@ -264,6 +279,17 @@ def main():
             "launches_per_epoch": n_launch,
             "ms_per_substep": round(per_substep_ms, 5),
             "tuning": tun,
+        },
+        "mc_roofline": {
+            "kernel": "k_mc (photon transport)",
+            "bound": "atomics",
+            "achieved": round(mc_atomic_rate, 2),
+            "peak": max(probe.values()),
+            "unit": "G atomic adds/s (measured probe)",
+            "frac": round(mc_atomic_rate / max(probe.values()), 4),
+            "atomics_per_launch": round(p["atomics"] / args.steps),
+            "avg_launch_ms": round(mc_avg, 4),
+            "probe": probe,
         },
     }
     if s:

@@ -80,6 +80,7 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         out->transmission = mc.transmitted / N;
         out->lateral_loss = mc.lateral / N;
         out->mc_events = mc.events;
+        out->mc_tally_atomics = mc.tally_atomics;
         out->t_mc_kernel_ms = mc.kernel_ms;
         double qsum = 0.0;
         BD_TRY(reduce_masked(c, c->Q, c->nvox, c->flags, F_ABL, true, &qsum));
@@ -397,6 +398,7 @@ extern "C" int bd_epoch_graph_finish(bd_ctx *h, bd_epoch_diag *out)
         out->transmission = m[2] / N;
         out->lateral_loss = m[3] / N;
         out->mc_events = m[5];
+        out->mc_tally_atomics = m[6];
         out->absorbed_J = r[40] * V * dt_epoch;
     } else {
         out->mc_energy_closure = 1.0;

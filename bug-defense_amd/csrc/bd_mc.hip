@@ -20,7 +20,9 @@
 // coalesces consecutive contributions to the same voxel in registers before
 // issuing them (one atomic pair per voxel visit, not per event).  The scalar
 // tallies (specular, diffuse, transmitted, lateral, absorbed, events) are kept
-// per lane in FP64 and reduced once per wave.
+// per lane in FP64 and reduced once per wave, with the count of voxel-tally
+// atomics issued (the transport's atomic rate is reported against a measured
+// ceiling, bd_probe_atomic_rate).
 #include "bd_internal.hpp"
 
 #include <chrono>
@@ -31,7 +33,7 @@ namespace bd {
 struct McArgs {
     const float4 *__restrict__ opt;   // {mu_a, mu_s, g, n}
     float2 *__restrict__ AL;
-    double *__restrict__ acc;         // 0 spec, 1 diffuse, 2 trans, 3 lateral, 4 absorbed, 5 events
+    double *__restrict__ acc;         // 0 spec, 1 diffuse, 2 trans, 3 lateral, 4 absorbed, 5 events, 6 tally atomics
     double *__restrict__ lat;
     unsigned long long *__restrict__ counter;
     int Nx, Ny, Nz;
@@ -161,12 +163,19 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
     float pA = 0.f, pL = 0.f;
     double t_spec = 0.0, t_diff = 0.0, t_trans = 0.0, t_lat = 0.0, t_abs = 0.0;
     long long t_ev = 0;
+    unsigned t_at = 0;  // voxel-tally atomics this lane issued
 
     auto flush = [&]() {
         if (pv >= 0) {
             float *cell = (float *)&a.AL[pv];
-            if (pA != 0.f) unsafeAtomicAdd(cell, pA);
-            if (a.score_L && pL != 0.f) unsafeAtomicAdd(cell + 1, pL);
+            if (pA != 0.f) {
+                unsafeAtomicAdd(cell, pA);
+                ++t_at;
+            }
+            if (a.score_L && pL != 0.f) {
+                unsafeAtomicAdd(cell + 1, pL);
+                ++t_at;
+            }
         }
         pv = -1;
         pA = 0.f;
@@ -472,9 +481,9 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
     }
     flush();
     // ---- wave reduction of the FP64 scalar tallies, one atomic per wave
-    double vals[6] = {t_spec, t_diff, t_trans, t_lat, t_abs, (double)t_ev};
+    double vals[7] = {t_spec, t_diff, t_trans, t_lat, t_abs, (double)t_ev, (double)t_at};
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
+    for (int q = 0; q < 7; ++q) {
         double v = vals[q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
         if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.acc[q], v);
@@ -743,7 +752,7 @@ int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n)
 int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
 {
     BD_TRY(launch_mc_enqueue(c, b, nullptr, 1));
-    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;
     out->specular = r[0];
@@ -752,6 +761,7 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
     out->lateral = r[3];
     out->absorbed = r[4];
     out->events = r[5];
+    out->tally_atomics = r[6];
     float ms = 0.f;
     BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     out->kernel_ms = ms;  // the transport kernel alone
@@ -759,3 +769,79 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
 }
 
 }  // namespace bd
+
+// ------------------------------------------------------------------ atomic-rate probe
+namespace bd {
+
+// The ceiling the transport's tally rate is set against: float atomic adds on
+// the (A, L) pair of one voxel of a float2-per-voxel grid, at voxel indices
+// hashed uniformly over [0, span).  Each lane issues its pairs back to back with
+// no dependent load between them, so the launch measures the memory system's
+// atomic throughput for that address spread rather than a latency chain.
+constexpr int PROBE_PAIRS_PER_LANE = 16;
+
+__global__ __launch_bounds__(256) void k_atomic_probe(float2 *__restrict__ al, uint64_t span, uint32_t seed)
+{
+    uint32_t h = ((blockIdx.x * 256u + threadIdx.x + 1u) * 0x9E3779B9u) ^ seed;
+    if (h == 0u) h = 1u;
+#pragma unroll
+    for (int i = 0; i < PROBE_PAIRS_PER_LANE; ++i) {
+        h ^= h << 13;  // xorshift32
+        h ^= h >> 17;
+        h ^= h << 5;
+        const uint64_t v = ((uint64_t)h * span) >> 32;  // < span
+        float *cell = (float *)&al[v];
+        unsafeAtomicAdd(cell, 1.0f);
+        unsafeAtomicAdd(cell + 1, 1.0f);
+    }
+}
+
+}  // namespace bd
+
+extern "C" int bd_probe_atomic_rate(int device, int64_t n_voxels, int64_t span, int64_t n_pairs, double *kernel_ms)
+{
+    using namespace bd;
+    BD_CHECK(kernel_ms, "bd_probe_atomic_rate: null argument");
+    BD_CHECK(n_voxels > 0 && n_voxels <= (int64_t(1) << 31), "bd_probe_atomic_rate: n_voxels out of range");
+    BD_CHECK(span > 0 && span <= n_voxels, "bd_probe_atomic_rate: span must be in [1, n_voxels]");
+    const int64_t per_block = 256 * PROBE_PAIRS_PER_LANE;
+    BD_CHECK(n_pairs > 0 && n_pairs % per_block == 0 && n_pairs / per_block <= (int64_t(1) << 20),
+             "bd_probe_atomic_rate: n_pairs must be a positive multiple of 4096, at most 2^32");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("bd_probe_atomic_rate: no HIP device");
+        return BD_ENODEV;
+    }
+    BD_CHECK(device >= 0 && device < ndev, "bd_probe_atomic_rate: device index out of range");
+    BD_HIP(hipSetDevice(device));
+    float2 *al = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e[2] = {nullptr, nullptr};
+    int rc = BD_OK;
+    auto run = [&]() -> int {
+        BD_HIP(hipMalloc(&al, sizeof(float2) * (size_t)n_voxels));
+        BD_HIP(hipStreamCreate(&s));
+        BD_HIP(hipEventCreate(&e[0]));
+        BD_HIP(hipEventCreate(&e[1]));
+        BD_HIP(hipMemsetAsync(al, 0, sizeof(float2) * (size_t)n_voxels, s));
+        const unsigned nb = (unsigned)(n_pairs / per_block);
+        // one untimed launch brings the grid into the caches the timed one sees
+        hipLaunchKernelGGL(k_atomic_probe, dim3(nb), dim3(256), 0, s, al, (uint64_t)span, 0x2545F491u);
+        BD_HIP(hipGetLastError());
+        BD_HIP(hipEventRecord(e[0], s));
+        hipLaunchKernelGGL(k_atomic_probe, dim3(nb), dim3(256), 0, s, al, (uint64_t)span, 0x9E3779B9u);
+        BD_HIP(hipGetLastError());
+        BD_HIP(hipEventRecord(e[1], s));
+        BD_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        BD_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+        *kernel_ms = ms;
+        return BD_OK;
+    };
+    rc = run();
+    if (e[0]) (void)hipEventDestroy(e[0]);
+    if (e[1]) (void)hipEventDestroy(e[1]);
+    if (s) (void)hipStreamDestroy(s);
+    if (al) (void)hipFree(al);
+    return rc;
+}

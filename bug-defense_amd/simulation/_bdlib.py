@@ -78,7 +78,8 @@ class Beam(ctypes.Structure):
 
 class MCResult(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("specular", "diffuse", "transmitted", "lateral", "absorbed", "events", "kernel_ms")]
+                ("specular", "diffuse", "transmitted", "lateral", "absorbed", "events", "kernel_ms",
+                 "tally_atomics")]
 
 
 class EpochDiag(ctypes.Structure):
@@ -86,7 +87,7 @@ class EpochDiag(ctypes.Structure):
         "absorbed_frac", "mc_energy_closure", "plume_tau", "plume_transmission", "plume_peak_mu_a",
         "plume_mean_mu_a", "t_optical_ms", "t_thermal_ms", "t_phase_ms", "absorbed_J", "reflected_J",
         "transmitted_J", "lateral_J", "conducted_J", "ablated_J", "R_spec", "R_diff", "transmission",
-        "lateral_loss", "mc_events", "t_substeps_ms", "t_mc_kernel_ms")]
+        "lateral_loss", "mc_events", "t_substeps_ms", "t_mc_kernel_ms", "mc_tally_atomics")]
 
 
 _lib = None
@@ -119,6 +120,8 @@ _SIGS = {
     "bd_plume_substeps": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "bd_mc_run": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(MCResult)], ctypes.c_int),
     "bd_mc_lateral": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_probe_atomic_rate": ([ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bd_phase_update": ([ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "bd_plume_void": ([ctypes.c_void_p], ctypes.c_int),
     "bd_plume_inject": ([ctypes.c_void_p], ctypes.c_int),
@@ -597,3 +600,15 @@ def scratch_context(shape, spacing) -> Context:
         ctx = Context(*key)
         _cache[key] = ctx
     return ctx
+
+
+def probe_atomic_rate(n_voxels, span=None, n_pairs=1 << 26, device=0):
+    """Float atomic adds per second that the device sustains on (A, L) voxel
+    pairs hashed uniformly over `span` voxels of an n_voxels float2 grid
+    (bd_probe_atomic_rate): the ceiling for the transport's tally atomics."""
+    span = int(n_voxels if span is None else span)
+    n_pairs = int(n_pairs) // 4096 * 4096
+    ms = ctypes.c_double(0.0)
+    check(lib().bd_probe_atomic_rate(int(device), int(n_voxels), span, n_pairs, ctypes.byref(ms)),
+          "bd_probe_atomic_rate")
+    return 2.0 * n_pairs / (ms.value * 1e-3)

@@ -117,6 +117,7 @@ typedef struct {
     double specular, diffuse, transmitted, lateral, absorbed; /* weight sums */
     double events;                 /* transport events traced */
     double kernel_ms;              /* device time of the transport kernel */
+    double tally_atomics;          /* float atomic adds issued into the A/L voxel tallies */
 } bd_mc_result;
 
 /* Per-epoch diagnostics (coupled_ablation.py:605-617) plus FP64 ledgers. */
@@ -130,6 +131,7 @@ typedef struct {
     double mc_events;
     double t_substeps_ms;          /* device time of the fused thermal sub-step launches alone */
     double t_mc_kernel_ms;         /* device time of the transport kernel alone */
+    double mc_tally_atomics;       /* float atomic adds the transport issued into the A/L tallies */
 } bd_epoch_diag;
 
 /* Device / library */
@@ -185,6 +187,13 @@ int bd_plume_substeps(bd_ctx *ctx, int n, double dt, double decay);   /* plume.p
 int bd_mc_run(bd_ctx *ctx, const bd_beam *beam, bd_mc_result *out);
 /* Lateral escape per depth index (voxel_mc_3d.py:697), n_depth values. */
 int bd_mc_lateral(bd_ctx *ctx, double *host, int64_t n_depth);
+
+/* Measurement helper, no counterpart in the reference: the ceiling for the
+ * transport's tally atomics.  One launch of n_pairs independent float atomic
+ * add pairs (A and L of one voxel) at voxel indices hashed uniformly over
+ * [0, span) of a float2 grid of n_voxels, on device `device`; writes the
+ * launch's device time.  n_pairs is a multiple of 4096 (at most 2^32). */
+int bd_probe_atomic_rate(int device, int64_t n_voxels, int64_t span, int64_t n_pairs, double *kernel_ms);
 
 /* Phase update + removal: _update_phase_properties_numba or
  * _update_single_stage_ablation_numba, then _remove with air-gap perforation

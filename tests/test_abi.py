@@ -33,8 +33,8 @@ def test_struct_sizes_match_header():
     """The ctypes mirrors must have the C layout (a mismatch corrupts every call)."""
     from simulation import _bdlib as B
     assert ctypes.sizeof(B.Grid) == 3 * 4 + 4 + 3 * 8  # int32 x3, pad, double x3
-    assert ctypes.sizeof(B.MCResult) == 7 * 8
-    assert ctypes.sizeof(B.EpochDiag) == 8 + 22 * 8
+    assert ctypes.sizeof(B.MCResult) == 8 * 8
+    assert ctypes.sizeof(B.EpochDiag) == 8 + 23 * 8
     assert ctypes.sizeof(B.FnoSpec) == 4 * 4
 
 
@@ -48,6 +48,18 @@ def test_no_device_raises_loudly():
         run_voxel_mc3d(n_photons=100, Nx=4, Ny=4, Nz=4)
     with pytest.raises(RuntimeError):
         B.Context((4, 4, 4), (0.1, 0.1, 0.1))
+
+
+def test_atomic_probe_rejects_bad_arguments():
+    """Argument checks run before any device call, so they hold without a GPU."""
+    from simulation import _bdlib as B
+    ms = ctypes.c_double(0.0)
+    for args in ((0, 100, 0, 4096), (0, 100, 101, 4096), (0, 100, 100, 4095), (0, 0, 0, 4096),
+                 (0, 100, 100, 0)):
+        rc = B.lib().bd_probe_atomic_rate(*args, ctypes.byref(ms))
+        assert rc == B.BD_EINVAL
+    with pytest.raises(ValueError):
+        B.probe_atomic_rate(100, span=200)
 
 
 def test_cpu_engine_keyword_rejected():

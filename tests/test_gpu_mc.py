@@ -143,3 +143,31 @@ def test_launch_axis_x_and_r_max_gpu(oracle):
     for key in ('R_diff', 'Absorption', 'Transmission', 'Lateral_loss'):
         assert abs(g['spectro_raw'][key] - c['spectro_raw'][key]) < 0.005, key
     assert g['lateral_escape_z'].sum() == pytest.approx(g['spectro_raw']['Lateral_loss'], rel=1e-5)
+
+
+def test_tally_atomic_count_and_probe():
+    """The transport's count of tally atomics: the same photons with and without
+    the path-length tally issue the same absorbed-weight atomics, and the
+    path-length atomics (one per voxel visit with a path) number at least as
+    many and at most one per event and photon.  The atomic-rate probe runs."""
+    from simulation import _bdlib as B
+    from simulation.voxel_mc_3d import make_beam
+    shape = (40, 40, 30)
+    ctx = B.Context(shape, (0.025, 0.025, 0.025))
+    try:
+        for name, v in (('MU_A', 0.5), ('MU_S', 12.0), ('G', 0.85), ('N', 1.4)):
+            ctx.upload(name, np.full(shape, v, dtype=np.float32))
+        res = {}
+        for sf in (0, 1):
+            beam = make_beam(1.0, 20_000, 0.3, 15.0, 0.0, (0.5, 0.5, 0.0), 0.0, 0, 'gaussian', 4242,
+                             score_fluence=bool(sf))
+            res[sf] = ctx.mc_run(beam)
+        n_a = res[0].tally_atomics
+        n_l = res[1].tally_atomics - n_a
+        assert res[0].events == res[1].events
+        assert n_a > 0 and n_l >= n_a
+        assert n_l <= res[1].events + 20_000
+    finally:
+        ctx.close()
+    rate = B.probe_atomic_rate(int(np.prod(shape)), n_pairs=4096 * 64)
+    assert rate > 1e8
