@@ -117,12 +117,13 @@ def measured_traffic(tuning):
     return traffic if tile == (tuning["substep_tile_y"], tuning["substep_tile_x"]) else None
 
 
-def run_surrogate(kw, steps, warmup, weights):
+def run_surrogate(kw, steps, warmup, weights, mc_lo, correct_block):
     import numpy as np
     from simulation.coupled_ablation import CoupledAblationEngine
-    from surrogate.coupled import SurrogateCoupledEngine
+    from surrogate.coupled import SurrogateCoupledEngine, excess
     eng = CoupledAblationEngine(**kw)
-    sur = SurrogateCoupledEngine(eng, weights, judge_every=10, tol=0.05)
+    sur = SurrogateCoupledEngine(eng, weights, judge_every=10, tol=0.05, mc_photons_lo=mc_lo,
+                                 correct_block=correct_block)
     for _ in range(warmup):
         sur.step_epoch()
     log(f"bench: surrogate loop, {warmup} warmup epochs done")
@@ -136,10 +137,14 @@ def run_surrogate(kw, steps, warmup, weights):
     wall = time.perf_counter() - t0
     hist = sur.history
     judged = [h["judge_error"] for h in hist if h["judge_error"] is not None]
+    mm = [h["mc_vs_mc"] for h in hist if h["judge_error"] is not None]
     timed = hist[warmup:]
     out = {"wall": wall, "surrogate_epochs": sum(1 for h in timed if h["optics_source"] == "surrogate"),
            "mc_epochs": sum(1 for h in timed if h["optics_source"] == "mc"),
            "judged_block4": [round(e[1], 4) for e in judged], "judged_voxel": [round(e[0], 4) for e in judged],
+           "mc_vs_mc_block4": [None if m is None else round(m, 4) for m in mm],
+           "excess_block4": [None if m is None else round(excess(e[1], m), 4) for e, m in zip(judged, mm)],
+           "judge_passes": sum(1 for h in hist if h["judge_error"] is not None and not h["fallback"]),
            "scale_after_judges": [round(h["scale"], 4) for h in hist if h["judge_error"] is not None],
            "state": final_state(eng, 21)}
     sur.close()
@@ -157,6 +162,9 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each plain epoch as a captured HIP graph")
     ap.add_argument("--plain-only", action="store_true", help="time the plain MC loop only")
     ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--mc-lo", type=int, default=37_500,
+                    help="photons of the low-photon MC that sets the surrogate's 8^3-block totals (0: none)")
+    ap.add_argument("--correct-block", type=int, default=8)
     ap.add_argument("--copy-kernel", action="store_true",
                     help="only run three 256 MiB device-to-device copies (calibrates the HBM byte counters)")
     ap.add_argument("--mc-probe", default=None, metavar="N1,N2,...",
@@ -233,7 +241,7 @@ def main():
     if not args.plain_only:
         if not os.path.exists(args.weights):
             raise RuntimeError(f"bench: surrogate weights not found at {args.weights}")
-        s = run_surrogate(kw, args.steps, args.warmup, args.weights)
+        s = run_surrogate(kw, args.steps, args.warmup, args.weights, args.mc_lo, args.correct_block)
     rate = args.steps * p["dt_epoch"] * 1e3 / (s["wall"] if s else p["wall"])
     wall = s["wall"] if s else p["wall"]
     result = {
@@ -253,8 +261,13 @@ def main():
             "workload": "coupled layered stack: 256x256x192 voxels of 15 um, 7 layers + 21 voxels ambient air, "
                         "40 W at 22 deg, 0.5 mm spot, 1 ms epochs of 150k photons + 20 x 50 us sub-steps, boil "
                         "100 C, ablate 350 C, Arrhenius char, diffusion plume on 128^3" +
-                        ("" if args.plain_only else "; surrogate on up to four 64^3 windows tiled along the beam, "
-                                                    "MC judge every 10th epoch, fallback above 5% L2"),
+                        ("" if args.plain_only else
+                         "; surrogate on up to four 64^3 windows tiled along the beam" +
+                         (f", its {args.correct_block}^3-block totals from a {args.mc_lo}-photon MC each epoch"
+                          if args.mc_lo else "") +
+                         "; judge every 10th epoch: the epoch's MC and a second independent MC of the same state, "
+                         "fallback when the surrogate-vs-MC L2 exceeds the MC-vs-MC L2 by more than 5% in "
+                         "quadrature"),
             "voxels": p["nvox"], "photons_per_epoch": p["n_photons"], "substeps_per_epoch": p["n_sub"],
             "mode": "plain MC" if args.plain_only else "surrogate+MC",
         },
@@ -297,6 +310,10 @@ def main():
         result["surrogate"] = {"value": round(rate, 3), "speedup_vs_plain": round(rate / plain_rate, 3),
                                "surrogate_epochs": s["surrogate_epochs"], "mc_epochs": s["mc_epochs"],
                                "judged_rel_l2_block4": jb, "judged_rel_l2_voxel": s["judged_voxel"],
+                               "mc_vs_mc_block4": s["mc_vs_mc_block4"], "excess_block4": s["excess_block4"],
+                               "judge_passes": s["judge_passes"], "judgements": len(jb),
+                               "gate": "d_s^2 <= d_mm^2 + 0.05^2 (block-4 rel L2; d_mm: a second independent "
+                                       "MC of the same state)",
                                "mc_anchored_scale": s["scale_after_judges"],
                                "max_judged_rel_l2_block4": max(jb) if jb else None, **s["state"]}
     if not args.no_cpu_baseline:

@@ -171,7 +171,8 @@ int bd_destroy(bd_ctx *h)
     void *bufs[] = {c->T, c->T2, c->H, c->Om, c->chi, c->Q, c->k, c->rc, c->hlat, c->flags, c->mu_a, c->mu_s,
                     c->gg, c->nn, c->wet_mu_a, c->wet_k, c->wet_rc, c->Tpeak, c->abl_time, c->AL, c->optics,
                     c->P, c->P2, c->Pext, c->pvoid, c->to_remove, c->red, c->lat, c->counter, c->staging,
-                    c->splat_tmp1, c->splat_tmp2, c->kb[0], c->kb[1], c->kb[2], c->kl[0], c->kl[1], c->kl[2], c->Qs, c->Qsur, c->Hb, c->flagsb};
+                    c->splat_tmp1, c->splat_tmp2, c->kb[0], c->kb[1], c->kb[2], c->kl[0], c->kl[1], c->kl[2],
+                    c->Qs, c->Qsur, c->Qaux, c->Hb, c->flagsb};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &f : c->ds)

@@ -71,6 +71,7 @@ struct Ctx {
     float4 *optics = nullptr;            // packed transport optics {mu_a, mu_s, g, n}
     float *Qs = nullptr;                 // surrogate coupling: Q of the last MC epoch
     float *Qsur = nullptr;               // surrogate coupling: composed field for judging
+    float *Qaux = nullptr;               // surrogate coupling: a second MC field (coarse amplitudes, MC-vs-MC)
     float *P = nullptr, *P2 = nullptr, *Pext = nullptr;  // plume grid
     uint8_t *pvoid = nullptr;
     uint8_t *to_remove = nullptr;

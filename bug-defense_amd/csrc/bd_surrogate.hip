@@ -121,6 +121,8 @@ struct FeatArgs {
     int sdir;
     float xc, yc, w0, n_amb;   // beam axis at the domain top
     float f5, f6;
+    const float2 *AL;          // when set, channel 5 is the log fluence of the context's last MC
+    float l_scale;             // path-length tally -> fluence relative to the beam's peak irradiance
     float *x;                  // [nwin][64][64][64][8]
 };
 
@@ -191,7 +193,12 @@ __global__ __launch_bounds__(FX) void k_sur_features(FeatArgs a)
             if (k >= 0 && t >= tw && t < tw + SW) {
                 float *xp = xw + (((int64_t)(t - tw) * SW + j) * SW + k) * 8;
                 const float4 f0 = make_float4(log1pf(op.x), log1pf(op.y * 0.1f), op.z, op.w - 1.f);
-                const float4 f1 = make_float4(logf(fmaxf(B, 1e-6f)) * (1.f / 13.815510557964274f) + 1.f, a.f5, a.f6,
+                float f5 = a.f5;
+                if (a.AL) {
+                    const float phi = (col_in && Z < g.Nz) ? a.AL[colv + Z].y * a.l_scale : 0.f;
+                    f5 = logf(phi + 1e-4f) * (1.f / 9.210340371976184f) + 1.f;
+                }
+                const float4 f1 = make_float4(logf(fmaxf(B, 1e-6f)) * (1.f / 13.815510557964274f) + 1.f, f5, a.f6,
                                               log1pf(tau) * (1.f / 4.61512051684126f));
                 reinterpret_cast<float4 *>(xp)[0] = f0;
                 reinterpret_cast<float4 *>(xp)[1] = f1;
@@ -313,6 +320,52 @@ __global__ void k_copy_f32(const float *__restrict__ s, float *__restrict__ d, i
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         d[i] = s[i];
+}
+
+// Coarse amplitudes from a second field: per block of e^3 voxels (e <= 8), the
+// non-ablated voxels of F are scaled by sum R / sum F over the block, when both
+// sums are positive (otherwise F is left as it is).  One workgroup per block,
+// which owns its voxels, so the sums and the scaling share one pass.
+__global__ __launch_bounds__(256) void k_sur_correct(float *__restrict__ F, const float *__restrict__ R,
+                                                     const uint8_t *__restrict__ flags, int Nx, int Ny, int Nz,
+                                                     int e)
+{
+    __shared__ double sf[256], sr[256];
+    const int nbz = (Nz + e - 1) / e, nby = (Ny + e - 1) / e;
+    const int bz = (int)(blockIdx.x % nbz), by = (int)((blockIdx.x / nbz) % nby), bx = (int)(blockIdx.x / (nbz * nby));
+    const int n = e * e * e;  // <= 512: two voxels per thread
+    int64_t vv[2];
+    bool ok[2];
+    float f[2], r[2];
+    double tf = 0.0, tr = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int l = threadIdx.x + 256 * h;
+        const int X = bx * e + l / (e * e), Y = by * e + (l / e) % e, Z = bz * e + l % e;
+        ok[h] = l < n && X < Nx && Y < Ny && Z < Nz;
+        vv[h] = ok[h] ? ((int64_t)X * Ny + Y) * Nz + Z : 0;
+        ok[h] = ok[h] && !(flags[vv[h]] & F_ABL);
+        f[h] = ok[h] ? F[vv[h]] : 0.f;
+        r[h] = ok[h] ? R[vv[h]] : 0.f;
+        tf += f[h];
+        tr += r[h];
+    }
+    sf[threadIdx.x] = tf;
+    sr[threadIdx.x] = tr;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sf[threadIdx.x] += sf[threadIdx.x + o];
+            sr[threadIdx.x] += sr[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    const double SF = sf[0], SR = sr[0];
+    if (!(SF > 0.0 && SR > 0.0)) return;  // block-uniform
+    const float ratio = (float)(SR / SF);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        if (ok[h]) F[vv[h]] = f[h] * ratio;
 }
 
 // judge error: per-block partials of sum (a - b)^2, sum a^2 over non-ablated
@@ -485,11 +538,14 @@ int bd_sur_window(bd_ctx *h, const bd_beam *beam, int margin, int32_t *origin)
     return BD_OK;
 }
 
-int bd_sur_features_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins, float *x_dev)
+int bd_sur_features_mc(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins, float *x_dev,
+                       int64_t mc_photons)
 {
     BD_CHECK(h && beam && origins && x_dev, "bd_sur_features: null argument");
     BD_CHECK(n >= 1 && n <= MAXW, "bd_sur_features: window count must be in [1, 4]");
+    BD_CHECK(mc_photons >= 0, "bd_sur_features: mc_photons must be >= 0");
     Ctx *c = &h->c;
+    BD_CHECK(mc_photons == 0 || c->AL, "bd_sur_features: no MC tallies on the context (run bd_mc_run first)");
     BD_TRY(ensure_state(c));
     for (int k = 0; k < n; ++k) BD_TRY(check_origin(c, origins + 4 * k));
     BeamFrame f;
@@ -507,15 +563,25 @@ int bd_sur_features_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *orig
     a.n_amb = (float)beam->n_ambient;
     a.f5 = f.sin0;
     a.f6 = (float)(beam->spot_dia / 0.5 - 1.0);
+    if (mc_photons > 0) {
+        a.AL = c->AL;
+        a.l_scale = (float)(M_PI * (double)f.w0 * f.w0 / 2.0 /
+                            ((double)mc_photons * c->g.dx * c->g.dy * c->g.dz));
+    }
     a.x = x_dev;
     hipLaunchKernelGGL(k_sur_features, dim3(SW, n), dim3(FX), 0, c->stream, a);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
 
+int bd_sur_features_n(bd_ctx *h, const bd_beam *beam, int n, const int32_t *origins, float *x_dev)
+{
+    return bd_sur_features_mc(h, beam, n, origins, x_dev, 0);
+}
+
 int bd_sur_features(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float *x_dev)
 {
-    return bd_sur_features_n(h, beam, 1, origin, x_dev);
+    return bd_sur_features_mc(h, beam, 1, origin, x_dev, 0);
 }
 
 int bd_sur_targets(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float *t_dev, float *w_dev)
@@ -613,6 +679,58 @@ int bd_sur_judge(bd_ctx *h, double *out)
     constexpr int NB = 256;
     double *part = c->red + 64;
     hipLaunchKernelGGL(k_sur_err, dim3(NB), dim3(256), 0, c->stream, c->Q, c->Qsur, c->flags, c->g.nx, c->g.ny,
+                       c->g.nz, part);
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sur_err_final, dim3(1), dim3(64), 0, c->stream, part, NB, c->red + 48);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 48, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    const double *r = (const double *)c->host_pinned;
+    out[0] = r[1] > 0.0 ? std::sqrt(r[0] / r[1]) : 0.0;
+    out[1] = r[3] > 0.0 ? std::sqrt(r[2] / r[3]) : 0.0;
+    out[2] = r[4];
+    out[3] = r[5];
+    return BD_OK;
+}
+
+int bd_sur_keep_aux(bd_ctx *h)
+{
+    BD_CHECK(h, "bd_sur_keep_aux: null argument");
+    Ctx *c = &h->c;
+    BD_TRY(ensure_state(c));
+    BD_TRY(ensure(c, (void **)&c->Qaux, (size_t)c->nvox * 4));
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_copy_f32, dim3(nb), dim3(256), 0, c->stream, c->Q, c->Qaux, c->nvox);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int bd_sur_correct(bd_ctx *h, int block, int into_q)
+{
+    BD_CHECK(h, "bd_sur_correct: null argument");
+    BD_CHECK(block >= 1 && block <= 8, "bd_sur_correct: block edge must be in [1, 8]");
+    Ctx *c = &h->c;
+    BD_CHECK(c->Qaux, "bd_sur_correct: no kept MC field (bd_sur_keep_aux)");
+    float *F = into_q ? c->Q : c->Qsur;
+    BD_CHECK(F && c->flags, "bd_sur_correct: no composed field (bd_sur_apply_n)");
+    const int64_t nbx = (c->g.nx + block - 1) / block, nby = (c->g.ny + block - 1) / block,
+                  nbz = (c->g.nz + block - 1) / block;
+    const int64_t nb = nbx * nby * nbz;
+    BD_CHECK(nb < (int64_t(1) << 31), "bd_sur_correct: too many blocks");
+    hipLaunchKernelGGL(k_sur_correct, dim3((unsigned)nb), dim3(256), 0, c->stream, F, c->Qaux, c->flags, c->g.nx,
+                       c->g.ny, c->g.nz, block);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+int bd_sur_judge_aux(bd_ctx *h, double *out)
+{
+    BD_CHECK(h && out, "bd_sur_judge_aux: null argument");
+    Ctx *c = &h->c;
+    BD_CHECK(c->Qaux && c->Q, "bd_sur_judge_aux: no kept MC field (bd_sur_keep_aux)");
+    constexpr int NB = 256;
+    double *part = c->red + 64;
+    hipLaunchKernelGGL(k_sur_err, dim3(NB), dim3(256), 0, c->stream, c->Q, c->Qaux, c->flags, c->g.nx, c->g.ny,
                        c->g.nz, part);
     BD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sur_err_final, dim3(1), dim3(64), 0, c->stream, part, NB, c->red + 48);

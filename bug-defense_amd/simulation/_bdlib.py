@@ -162,10 +162,15 @@ _SIGS = {
                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "bd_sur_features_n": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                            ctypes.c_void_p], ctypes.c_int),
+    "bd_sur_features_mc": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                            ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "bd_sur_apply_n": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                         ctypes.c_void_p, ctypes.c_int, ctypes.c_double], ctypes.c_int),
     "bd_sur_judge": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bd_sur_keep_stale": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_sur_keep_aux": ([ctypes.c_void_p], ctypes.c_int),
+    "bd_sur_correct": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "bd_sur_judge_aux": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bd_sur_error": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     # surrogate network (bugdef_fno.h); tensors are device pointers
     "bd_fno_create": ([ctypes.c_int, ctypes.POINTER(FnoSpec), ctypes.c_int, ctypes.c_int,
@@ -509,12 +514,15 @@ class Context:
         flat = [int(v) for o in origins for v in (tuple(o[:3]) + (0,))]
         return (ctypes.c_int32 * len(flat))(*flat), len(origins)
 
-    def sur_features_n(self, beam: Beam, origins, x_dev):
-        """Features of len(origins) windows into x_dev [n, 64, 64, 64, 8]."""
+    def sur_features_n(self, beam: Beam, origins, x_dev, mc_photons=0):
+        """Features of len(origins) windows into x_dev [n, 64, 64, 64, 8]; with
+        mc_photons > 0 channel 5 is the fluence of the last MC run (of that many
+        photons) instead of sin(theta)."""
         o, n = self._origins(origins)
         if x_dev.shape[0] < n:
             raise ValueError("sur_features_n: x_dev holds fewer windows than origins")
-        check(lib().bd_sur_features_n(self.handle, ctypes.byref(beam), n, o, x_dev.ptr), "bd_sur_features_n")
+        check(lib().bd_sur_features_mc(self.handle, ctypes.byref(beam), n, o, x_dev.ptr, int(mc_photons)),
+              "bd_sur_features_mc")
 
     def sur_apply_n(self, beam: Beam, origins, y_dev, into_q, scale=1.0):
         """Compose Q from predictions y_dev [n, 64, 64, 64] of len(origins) windows,
@@ -537,6 +545,19 @@ class Context:
         """(per-voxel rel L2, block-4 rel L2, sum of Q, sum of the composed field)."""
         out = (ctypes.c_double * 4)()
         check(lib().bd_sur_judge(self.handle, out), "bd_sur_judge")
+        return tuple(float(v) for v in out)
+
+    def sur_keep_aux(self):
+        check(lib().bd_sur_keep_aux(self.handle), "bd_sur_keep_aux")
+
+    def sur_correct(self, block, into_q):
+        """Block totals of the composed field from the kept MC field (bd_sur_correct)."""
+        check(lib().bd_sur_correct(self.handle, int(block), 1 if into_q else 0), "bd_sur_correct")
+
+    def sur_judge_aux(self):
+        """sur_judge of the kept MC field against Q (the MC-vs-MC distance)."""
+        out = (ctypes.c_double * 4)()
+        check(lib().bd_sur_judge_aux(self.handle, out), "bd_sur_judge_aux")
         return tuple(float(v) for v in out)
 
     def reduce(self, name, op="sum", mask=None):

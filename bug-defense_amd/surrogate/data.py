@@ -10,6 +10,10 @@ coupled runs); the beam has a random angle in [0, 30] deg and spot in
 the window centre at mid-depth).  The target is the window's log fluence from
 an MC run of n_photons on the device; features and targets are written
 straight into device buffers (no host round trip of the 3-D fields).
+
+With mc_photons > 0 the samples are for the refining network: channel 5 of
+the features is the fluence of a separate mc_photons-photon MC of the same
+state (independent photon streams), and the target is still the n_photons run.
 """
 
 import math
@@ -72,6 +76,18 @@ def sample_beam(s, n_photons, seed, power=1.0):
                      'gaussian', seed, score_fluence=True)
 
 
+NOISY_PHOTON_OFFSET = 1 << 40   # photon ids of the low-photon input run: disjoint from the target run's
+
+
+def noisy_beam(beam, mc_photons, photon_offset=NOISY_PHOTON_OFFSET):
+    """The beam of an extra MC run of the same state: same key, disjoint photon ids."""
+    b = B.Beam.from_buffer_copy(beam)
+    b.n_photons = int(mc_photons)
+    b.photon_offset = int(photon_offset)
+    b.score_fluence = 1
+    return b
+
+
 class Dataset:
     """N samples resident in device memory: X [N,64,64,64,8], t and w [N,64,64,64]."""
 
@@ -88,7 +104,7 @@ class Dataset:
 
 
 def generate_coupled(n, n_photons=1_000_000, seed=0, shape=(192, 192, 192), epochs_per_run=150, snapshot_every=3,
-                     progress=None, device=0, max_windows=4):
+                     progress=None, device=0, max_windows=4, mc_photons=0):
     """Dataset of n windows taken from coupled runs of the layered-stack workload,
     i.e. the states the coupled loop hands the network: every `snapshot_every`-th
     epoch of a plain-MC run (random stack seed, power 20-80 W, angle 0-30 deg, spot
@@ -114,8 +130,11 @@ def generate_coupled(n, n_photons=1_000_000, seed=0, shape=(192, 192, 192), epoc
                     continue
                 beam = eng._epoch_beam()
                 ws = eng._ctx.sur_windows(beam, 12, max_windows)[:n - i]
+                if mc_photons:
+                    beam.seed = int(rng.integers(1, 1 << 62))
+                    eng._ctx.mc_run(noisy_beam(beam, mc_photons))
                 for k, o in enumerate(ws):
-                    eng._ctx.sur_features(beam, o, ds.X.view(i + k, (T, T, T, CIN)))
+                    eng._ctx.sur_features_n(beam, [o], ds.X.view(i + k, (1, T, T, T, CIN)), mc_photons)
                 beam.n_photons = int(n_photons)
                 beam.seed = int(rng.integers(1, 1 << 62))
                 beam.score_fluence = 1
@@ -137,7 +156,7 @@ def generate_coupled(n, n_photons=1_000_000, seed=0, shape=(192, 192, 192), epoc
     return ds
 
 
-def generate(n, n_photons=1_000_000, seed=0, progress=None, device=0, mc_seed_offset=0):
+def generate(n, n_photons=1_000_000, seed=0, progress=None, device=0, mc_seed_offset=0, mc_photons=0):
     """Dataset of n samples from the device MC (seeded, reproducible); the
     geometry depends on `seed` only, the MC streams also on mc_seed_offset."""
     rng = np.random.default_rng(seed)
@@ -150,8 +169,10 @@ def generate(n, n_photons=1_000_000, seed=0, progress=None, device=0, mc_seed_of
             for name in ('mu_a', 'mu_s', 'g', 'n'):
                 ctx.upload(name.upper(), s[name])
             beam = sample_beam(s, n_photons, seed * 1_000_003 + i + 1 + mc_seed_offset * 7_777_777_777)
+            if mc_photons:
+                ctx.mc_run(noisy_beam(beam, mc_photons))
+            ctx.sur_features_n(beam, [origin], ds.X.view(i, (1, T, T, T, CIN)), mc_photons)
             ctx.mc_run(beam)
-            ctx.sur_features(beam, origin, ds.X.view(i, (T, T, T, CIN)))
             ctx.sur_targets(beam, origin, ds.t.view(i, (T, T, T)), ds.w.view(i, (T, T, T)))
             ds.meta.append({k: s[k] for k in ('theta', 'spot', 'xc', 'yc')})
             if progress and (i + 1) % progress == 0:

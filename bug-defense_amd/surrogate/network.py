@@ -64,6 +64,7 @@ class FourierSurrogate:
         self.max_batch = int(max_batch)
         self.train = bool(train)
         self.device = int(device)
+        self.mc_photons = 0   # > 0: trained on features whose channel 5 is a run of this many photons
         spec = B.FnoSpec(self.modes[0], self.modes[1], self.modes[2], self.np_len)
         h = ctypes.c_void_p()
         B.check(B.lib().bd_fno_create(self.device, ctypes.byref(spec), self.max_batch, 1 if train else 0,
@@ -76,7 +77,7 @@ class FourierSurrogate:
     # -- parameters -----------------------------------------------------------
     def save(self, path, **meta):
         np.savez(path, params=self.get_params(), modes=np.array(self.modes), np_len=self.np_len,
-                 **{k: np.asarray(v) for k, v in meta.items()})
+                 mc_photons=self.mc_photons, **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
     def load(cls, path, max_batch=8, train=False, device=0):
@@ -84,6 +85,7 @@ class FourierSurrogate:
             net = cls(modes=tuple(int(v) for v in z["modes"]), np_len=int(z["np_len"]), max_batch=max_batch,
                       train=train, device=device)
             net.set_params(z["params"])
+            net.mc_photons = int(z["mc_photons"]) if "mc_photons" in z.files else 0
         return net
 
     def set_params(self, flat):

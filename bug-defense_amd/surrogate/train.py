@@ -20,6 +20,7 @@ from . import data
 from .network import FourierSurrogate, init_params, T
 
 PHI_EPS = 1e-4
+LN1E4 = math.log(1e4)
 
 
 def q_from(log_phi, feats_mu_a):
@@ -41,18 +42,23 @@ def evaluate(net, ds, idx0, n, batch, other=None):
     """Relative L2 of Q, prediction vs the sample's MC, on samples [idx0, idx0 + n):
     median per voxel, median over 4^3 blocks, pooled per voxel.  With `other`
     (the same samples under independent MC streams) also the MC-vs-MC medians."""
-    ev, eb, mv, mb, num, den = [], [], [], [], 0.0, 0.0
+    ev, eb, mv, mb, ib, num, den = [], [], [], [], [], 0.0, 0.0
     for c in range(idx0 // batch, (idx0 + n) // batch):
         X, t, _ = ds.chunk(c, batch)
         y = net.forward(X).download().astype(np.float64)
         tt = t.download().astype(np.float64)
-        mu = X.download()[..., 0].astype(np.float64)
+        xh = X.download()
+        mu = xh[..., 0].astype(np.float64)
+        # the refining network's input run, as absorbed power (channel 5 inverted)
+        qi = q_from((xh[..., 5].astype(np.float64) - 1.0) * LN1E4, mu) if net.mc_photons else None
         qp = q_from(y, mu)
         qt = q_from(tt, mu)
         qo = q_from(other.chunk(c - idx0 // batch, batch)[1].download().astype(np.float64), mu) if other else None
         for b in range(batch):
             ev.append(rel_l2(qp[b], qt[b]))
             eb.append(rel_l2(block4(qp[b]), block4(qt[b])))
+            if qi is not None:
+                ib.append(rel_l2(block4(qi[b]), block4(qt[b])))
             num += float(((qp[b] - qt[b]) ** 2).sum())
             den += float((qt[b] ** 2).sum())
             if other:
@@ -60,6 +66,8 @@ def evaluate(net, ds, idx0, n, batch, other=None):
                 mb.append(rel_l2(block4(qo[b]), block4(qt[b])))
     out = {"median_voxel": float(np.median(ev)), "median_block4": float(np.median(eb)),
            "pooled_voxel": math.sqrt(num / den)}
+    if ib:
+        out["input_run_median_block4"] = float(np.median(ib))
     if other:
         out.update(mc_vs_mc_median_voxel=float(np.median(mv)), mc_vs_mc_median_block4=float(np.median(mb)))
     return out
@@ -81,6 +89,8 @@ def main(argv=None):
     ap.add_argument("--coupled", type=int, default=0, help="windows taken from coupled runs (added to --samples)")
     ap.add_argument("--coupled-holdout", type=int, default=64)
     ap.add_argument("--np-len", type=int, default=80, help="transform length")
+    ap.add_argument("--mc-photons", type=int, default=0,
+                    help="train the refining network: channel 5 is the fluence of a run of this many photons")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     nb = a.batch
@@ -89,11 +99,12 @@ def main(argv=None):
     t0 = time.perf_counter()
     nco = (a.coupled // nb) * nb
     ncho = (a.coupled_holdout // nb) * nb if nco else 0
-    ds = data.generate(ntr, n_photons=a.photons, seed=a.seed, progress=1024) if ntr else None
-    ho = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000)
-    ho2 = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000, mc_seed_offset=1)
-    cds = data.generate_coupled(nco, n_photons=a.photons, seed=a.seed + 7, progress=512) if nco else None
-    cho = data.generate_coupled(ncho, n_photons=a.photons, seed=a.seed + 2000) if ncho else None
+    mp = a.mc_photons
+    ds = data.generate(ntr, n_photons=a.photons, seed=a.seed, progress=1024, mc_photons=mp) if ntr else None
+    ho = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000, mc_photons=mp)
+    ho2 = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000, mc_seed_offset=1, mc_photons=mp)
+    cds = data.generate_coupled(nco, n_photons=a.photons, seed=a.seed + 7, progress=512, mc_photons=mp) if nco else None
+    cho = data.generate_coupled(ncho, n_photons=a.photons, seed=a.seed + 2000, mc_photons=mp) if ncho else None
     t_data = time.perf_counter() - t0
     print(f"data: {ntr} train + {nho} held-out samples (+ independent MC of the held-out) in {t_data:.1f} s",
           flush=True)
@@ -103,6 +114,7 @@ def main(argv=None):
         modes = tuple(int(v) for v in a.modes.split(","))
         net = FourierSurrogate(modes=modes, np_len=a.np_len, max_batch=nb, train=True)
         net.set_params(init_params(net.modes, seed=a.seed))
+    net.mc_photons = mp
     rng = np.random.default_rng(a.seed + 1)
     chunks = [(ds, c) for c in range(ntr // nb)] + [(cds, c) for c in range(nco // nb)]
     nchunk = len(chunks)
@@ -126,7 +138,8 @@ def main(argv=None):
         hist.append(rec)
         print(json.dumps(rec), flush=True)
     t_train = time.perf_counter() - t1
-    summary = {"train_samples": ntr, "heldout_samples": nho, "photons": a.photons, "epochs": a.epochs,
+    summary = {"train_samples": ntr, "heldout_samples": nho, "photons": a.photons, "mc_photons": mp,
+               "epochs": a.epochs,
                "modes": list(net.modes), "np_len": net.np_len, "data_s": round(t_data, 1),
                "train_s": round(t_train, 1), "heldout": evaluate(net, ho, 0, nho, nb, other=ho2),
                "coupled_windows": nco, "coupled_heldout": evaluate(net, cho, 0, ncho, nb) if cho else None}

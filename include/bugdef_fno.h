@@ -105,6 +105,15 @@ int bd_sur_features(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, flo
  * x_dev [n][64][64][64][8], in one launch. */
 int bd_sur_features_n(bd_ctx *ctx, const bd_beam *beam, int n, const int32_t *origins, float *x_dev);
 
+/* bd_sur_features_n, but with mc_photons > 0 channel 5 carries the fluence of
+ * the context's last MC run instead of sin(theta): ln(Phi/E0 + 1e-4)/ln(1e4) + 1,
+ * Phi = L / (mc_photons V), E0 the beam's peak irradiance per unit power (the
+ * scaling of bd_sur_targets), 0 outside the domain.  That run must have had
+ * mc_photons photons and score_fluence set.  A network trained on these
+ * features refines a low-photon MC field into the high-photon one. */
+int bd_sur_features_mc(bd_ctx *ctx, const bd_beam *beam, int n, const int32_t *origins, float *x_dev,
+                       int64_t mc_photons);
+
 /* Training target t = ln(Phi/E0 + 1e-4) and loss weights (sum one) of the
  * window from the context's last MC tallies (bd_mc_run with score_fluence). */
 int bd_sur_targets(bd_ctx *ctx, const bd_beam *beam, const int32_t *origin, float *t_dev, float *w_dev);
@@ -137,6 +146,23 @@ int bd_sur_error(bd_ctx *ctx, double *out);
  * relative L2 as bd_sur_error, out[2] the sum of Q and out[3] the sum of the
  * composed field, both over non-ablated voxels. */
 int bd_sur_judge(bd_ctx *ctx, double *out);
+
+/* A second kept field: copy the context's Q (typically just written by an MC
+ * run of its own) into it. */
+int bd_sur_keep_aux(bd_ctx *ctx);
+
+/* Coarse amplitudes from the kept MC field: for each block of block^3 voxels
+ * (block 1..8), the non-ablated voxels of the composed field (Q with
+ * into_q != 0, else the judged field) are scaled by sum kept / sum composed
+ * over the block's non-ablated voxels, when both sums are positive.  The
+ * network then supplies the shape inside each block and a low-photon MC the
+ * block totals. */
+int bd_sur_correct(bd_ctx *ctx, int block, int into_q);
+
+/* bd_sur_judge for the kept MC field against Q: with the kept field from an
+ * independent MC of the same state this is the MC-vs-MC distance, the shot
+ * noise any comparison with one MC run carries. */
+int bd_sur_judge_aux(bd_ctx *ctx, double *out);
 
 /* Plain device buffers for the tensors above, so a host program needs no other
  * GPU runtime: allocation on `device`, synchronous host<->device and

@@ -167,8 +167,13 @@ def ballistic(mu_a, mu_s, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_am
     return (out, tout) if with_tau else out
 
 
-def features(mu_a, mu_s, g, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0):
-    """[64, 64, 64, 8] float64, the device channel order."""
+LN1E4 = math.log(1e4)
+
+
+def features(mu_a, mu_s, g, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_amb=1.0, L=None, mc_photons=0):
+    """[64, 64, 64, 8] float64, the device channel order.  With a path-length
+    tally L of an mc_photons-photon run, channel 5 is that run's fluence,
+    ln(Phi/E0 + 1e-4)/ln(1e4) + 1 (the scaling of `targets`), instead of sin(theta)."""
     x = np.zeros((SW, SW, SW, 8))
     ma = _gather(mu_a, origin, 0.0)
     ms = _gather(mu_s, origin, 0.0)
@@ -180,7 +185,13 @@ def features(mu_a, mu_s, g, n, spacing, origin, xc, yc, theta, spot, phi=0.0, n_
     x[..., 2] = gg
     x[..., 3] = nn - 1.0
     x[..., 4] = np.log(np.maximum(B, 1e-6)) / LN1E6 + 1.0
-    x[..., 5] = math.sin(theta)
+    if L is not None and mc_photons > 0:
+        dx, dy, dz = spacing
+        w0 = spot / 2.0
+        scale = math.pi * w0 * w0 / 2.0 / (mc_photons * dx * dy * dz)
+        x[..., 5] = np.log(_gather(L, origin, 0.0) * scale + PHI_EPS) / LN1E4 + 1.0
+    else:
+        x[..., 5] = math.sin(theta)
     x[..., 6] = spot / 0.5 - 1.0
     x[..., 7] = np.log1p(tau) / math.log(101.0)
     return x
@@ -251,6 +262,25 @@ def apply(y, mu_a, Qs, origin, power, spot):
     qs = E0 * ma * np.maximum(np.exp(y[:ex, :ey, :ez]) - PHI_EPS, 0.0)
     rr = r[:ex, :ey, :ez]
     out[x0:x0 + ex, y0:y0 + ey, z0:z0 + ez] = rr * qs + (1.0 - rr) * out[x0:x0 + ex, y0:y0 + ey, z0:z0 + ez]
+    return out
+
+
+def coarse_correct(F, R, ablated, block):
+    """bd_sur_correct: per block^3 block, F's non-ablated voxels times
+    sum R / sum F over the block's non-ablated voxels, when both sums are > 0."""
+    out = np.array(F, dtype=np.float64)
+    keep = ~np.asarray(ablated, bool)
+    Nx, Ny, Nz = out.shape
+    for x in range(0, Nx, block):
+        for y in range(0, Ny, block):
+            for z in range(0, Nz, block):
+                sl = (slice(x, x + block), slice(y, y + block), slice(z, z + block))
+                m = keep[sl]
+                sf = float(out[sl][m].sum())
+                sr = float(np.asarray(R, np.float64)[sl][m].sum())
+                if sf > 0.0 and sr > 0.0:
+                    blk = out[sl]
+                    blk[m] *= sr / sf
     return out
 
 

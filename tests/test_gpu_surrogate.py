@@ -49,6 +49,68 @@ def test_features_and_targets_match_oracle_on_a_training_sample():
     c.close()
 
 
+def test_mc_fluence_channel_matches_oracle():
+    """bd_sur_features_mc: channel 5 carries the last MC run's fluence (the
+    target scaling), every other channel is that of bd_sur_features_n."""
+    from simulation._bdlib import DeviceArray
+    from surrogate import data as D
+    s = D.random_sample(np.random.default_rng(5))
+    c = _ctx(s, (64, 64, 64))
+    beam = D.sample_beam(s, 40_000, 17)
+    c.mc_run(beam)
+    x0, x1 = DeviceArray((1, 64, 64, 64, 8)), DeviceArray((1, 64, 64, 64, 8))
+    c.sur_features_n(beam, [(0, 0, 0)], x0)
+    c.sur_features_n(beam, [(0, 0, 0)], x1, mc_photons=40_000)
+    c.sync()
+    L = c.download('L')
+    xo = S.features(*(_f32(s[k]) for k in ('mu_a', 'mu_s', 'g', 'n')), SP, (0, 0, 0), s['xc'], s['yc'],
+                    s['theta'], s['spot'], L=L, mc_photons=40_000)
+    a, b = x0.download()[0], x1.download()[0]
+    for ch in (0, 1, 2, 3, 4, 6, 7):
+        np.testing.assert_array_equal(a[..., ch], b[..., ch])
+    np.testing.assert_allclose(b[..., 5], xo[..., 5], rtol=2e-5, atol=2e-5)
+    assert b[..., 5].max() > 0.5 and b[..., 5].min() >= 0.0
+    # exp of the channel recovers Phi/E0 (the target transform)
+    to, _ = S.targets(L, _f32(s['mu_a']), SP, (0, 0, 0), 40_000, s['spot'])
+    np.testing.assert_allclose((b[..., 5] - 1.0) * S.LN1E4, to, rtol=1e-4, atol=1e-4)
+    c.close()
+
+
+def test_coarse_correct_and_mc_vs_mc_judge_match_oracle():
+    """bd_sur_correct (block totals from the kept field) and bd_sur_judge_aux
+    against the oracle on a ragged grid with ablated voxels."""
+    from simulation import _bdlib as B
+    shape = (37, 30, 27)
+    rng = np.random.default_rng(12)
+    c = B.Context(shape, SP)
+    try:
+        abl = rng.random(shape) < 0.15
+        F = rng.gamma(2.0, 1.0, shape).astype(np.float32)
+        R = (F * rng.uniform(0.5, 1.5, shape)).astype(np.float32)
+        F[:8, :8, :8] = 0.0
+        c.upload('ABLATED', abl)
+        for block in (8, 4):
+            c.upload('Q', R)
+            c.sur_keep_aux()
+            c.upload('Q', F)
+            c.sur_correct(block, into_q=True)
+            got = c.download('Q').astype(np.float64)
+            want = S.coarse_correct(_f32(F), _f32(R), abl, block)
+            np.testing.assert_allclose(got, want, rtol=2e-6, atol=1e-12, err_msg=f"block {block}")
+        # MC-vs-MC distance: kept field against Q
+        c.upload('Q', R)
+        c.sur_keep_aux()
+        c.upload('Q', F)
+        j = c.sur_judge_aux()
+        e0, e1 = S.rel_error(_f32(F), _f32(R), abl)
+        assert abs(j[0] - e0) < 1e-9 and abs(j[1] - e1) < 1e-9
+        keep = ~abl
+        assert abs(j[2] - float(_f32(F)[keep].sum())) < 1e-6 * j[2]
+        assert abs(j[3] - float(_f32(R)[keep].sum())) < 1e-6 * j[3]
+    finally:
+        c.close()
+
+
 def _stack(shape, rng, air=21):
     props = {k: np.zeros(shape) for k in ('mu_a', 'mu_s', 'g')}
     props['n'] = np.ones(shape)

@@ -55,24 +55,30 @@ def _engine(**kw):
 def test_loop_judges_every_tenth_epoch_and_falls_back():
     from surrogate.coupled import SurrogateCoupledEngine
     eng = _engine()
-    sur = SurrogateCoupledEngine(eng, WEIGHTS, judge_every=5, tol=10.0)   # a tolerance nothing exceeds
+    # a tolerance nothing exceeds; the coarse amplitudes from a low-photon MC on every prediction
+    sur = SurrogateCoupledEngine(eng, WEIGHTS, judge_every=5, tol=10.0, mc_photons_lo=20_000)
     for _ in range(11):
         sur.step_epoch()
     src = [h["optics_source"] for h in sur.history]
     # epoch 0: MC (the field outside the window); 1: MC + first judgement; then every 5th
     assert src == ["mc", "mc"] + ["surrogate"] * 3 + ["mc"] + ["surrogate"] * 4 + ["mc"]
-    errs = [h["judge_error"] for h in sur.history if h["judge_error"] is not None]
-    assert len(errs) == 3 and all(0.0 <= e[1] < 10.0 for e in errs)
-    print("judged (voxel, block4):", errs)
+    judged = [h for h in sur.history if h["judge_error"] is not None]
+    assert len(judged) == 3 and all(0.0 <= h["judge_error"][1] < 10.0 for h in judged)
+    # the noise-aware judge measured the MC-vs-MC distance beside each judgement
+    assert all(0.0 < h["mc_vs_mc"] < 1.0 for h in judged)
+    print("judged (voxel, block4), mc-vs-mc:", [(h["judge_error"], h["mc_vs_mc"]) for h in judged])
     sur.close()
     eng.close()
-    eng = _engine()
-    sur = SurrogateCoupledEngine(eng, WEIGHTS, judge_every=5, tol=0.0)    # a tolerance everything exceeds
-    for _ in range(11):
-        sur.step_epoch()
-    assert [h["optics_source"] for h in sur.history] == ["mc"] * 11
-    assert all(h["fallback"] for h in sur.history[1:])
-    # while falling back, only the judge epochs (1, 5, 10) compare against the network
-    assert [i for i, h in enumerate(sur.history) if h["judge_error"] is not None] == [1, 5, 10]
-    sur.close()
-    eng.close()
+    for backoff, idx in ((True, [1, 6]), (False, [1, 5, 10])):
+        eng = _engine()
+        # a tolerance everything exceeds
+        sur = SurrogateCoupledEngine(eng, WEIGHTS, judge_every=5, tol=0.0, noise_aware=False, backoff=backoff)
+        for _ in range(11):
+            sur.step_epoch()
+        assert [h["optics_source"] for h in sur.history] == ["mc"] * 11
+        assert all(h["fallback"] for h in sur.history[1:])
+        # while falling back only judge epochs compare against the network: every 5th, or
+        # with backoff after 5, then 10 epochs
+        assert [i for i, h in enumerate(sur.history) if h["judge_error"] is not None] == idx
+        sur.close()
+        eng.close()

@@ -134,3 +134,64 @@ def test_apply_n_is_a_partition_of_unity_in_the_overlap():
     # one window reduces to apply()
     np.testing.assert_allclose(S.apply_n(ys[:1], ma, Qs, ws[:1], 40.0, 0.5), S.apply(ys[0], ma, Qs, ws[0], 40.0, 0.5),
                                rtol=1e-12)
+
+
+def test_coarse_correct_takes_block_totals_and_keeps_the_shape():
+    """Invariants that pin bd_sur_correct's restatement: block totals become R's,
+    the shape inside a block is F's, ablated voxels and blocks with an empty
+    sum are untouched."""
+    rng = np.random.default_rng(4)
+    shape, e = (13, 10, 9), 4
+    F = rng.uniform(0.0, 2.0, shape)
+    R = rng.uniform(0.0, 3.0, shape)
+    abl = rng.random(shape) < 0.2
+    F[8:, :4, :4] = 0.0            # a block whose composed sum is empty
+    R[:4, 4:8, 4:8] = 0.0          # a block whose MC sum is empty
+    out = S.coarse_correct(F, R, abl, e)
+    np.testing.assert_array_equal(out[abl], F[abl])
+    for x in range(0, shape[0], e):
+        for y in range(0, shape[1], e):
+            for z in range(0, shape[2], e):
+                sl = (slice(x, x + e), slice(y, y + e), slice(z, z + e))
+                m = ~abl[sl]
+                sf, sr = F[sl][m].sum(), R[sl][m].sum()
+                if sf > 0 and sr > 0:
+                    assert abs(out[sl][m].sum() - sr) <= 1e-12 * sr
+                    k = np.flatnonzero(F[sl][m] > 0)
+                    ratios = out[sl][m][k] / F[sl][m][k]
+                    np.testing.assert_allclose(ratios, ratios[0], rtol=1e-12)
+                else:
+                    np.testing.assert_array_equal(out[sl], F[sl])
+    np.testing.assert_array_equal(out[8:, :4, :4], 0.0)
+    np.testing.assert_array_equal(out[:4, 4:8, 4:8][~abl[:4, 4:8, 4:8]], F[:4, 4:8, 4:8][~abl[:4, 4:8, 4:8]])
+
+
+def test_noise_aware_gate_separates_noise_from_bias():
+    """E d_s^2 = e^2 + s^2 and E d_mm^2 = 2 s^2 (e: the candidate's distance from
+    the expected field, s: one MC run's noise).  On a synthetic field with
+    Poisson-like noise: a candidate that is itself an independent MC run of the
+    same noise passes (e = s), one 10 % off in amplitude fails, and the plain
+    L2 test fails both once s exceeds the tolerance."""
+    from surrogate.coupled import excess, gate_passes
+    rng = np.random.default_rng(9)
+    truth = rng.gamma(2.0, 1.0, 200_000)
+
+    def mc(sig):
+        return truth * (1.0 + sig * rng.standard_normal(truth.size))
+
+    def d(a, b):
+        return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+    for sig in (0.02, 0.08):
+        m1, m2 = mc(sig), mc(sig)
+        d_mm = d(m2, m1)
+        assert abs(d_mm - np.sqrt(2) * sig) < 0.05 * np.sqrt(2) * sig
+        cand_mc = mc(sig)
+        assert gate_passes(d(cand_mc, m1), d_mm, 0.05)
+        assert excess(d(cand_mc, m1), d_mm) < 0.6 * sig + 0.01
+        biased = 1.10 * truth
+        assert not gate_passes(d(biased, m1), d_mm, 0.05)
+        assert abs(excess(d(biased, m1), d_mm) - np.sqrt(0.10 ** 2 - sig ** 2)) < 0.02
+    # the plain test rejects an exact copy of the expected field when one run is noisy
+    m1 = mc(0.08)
+    assert not gate_passes(d(truth, m1), None, 0.05)
