@@ -95,10 +95,13 @@ def test_fused_ablation_parity_reference_fixture(oracle):
     assert H.max() > 0.0 and chi.max() > 0.0  # the fixture exercises both branches
 
 
-@pytest.mark.parametrize("shape", [(40, 37, 96), (128, 128, 96)])
+@pytest.mark.parametrize("shape", [(40, 37, 96), (128, 128, 96), (1, 1, 1), (1, 5, 7), (3, 1, 70), (2, 3, 1),
+                                   (5, 4, 2), (2, 2, 129)])
 def test_multi_tile_parity_with_plateau(oracle, shape):
     """Several z tiles (96 = 64 + 32, a partial tile), partial y tiles, the tuned tile shape
-    (>= 1M voxels) and every plateau branch, against the oracle over 20 sub-steps."""
+    (>= 1M voxels) and every plateau branch, against the oracle over 20 sub-steps; also
+    degenerate grids (one cell, one column, one plane, one row: both boundary fluxes
+    on one cell, no x or y neighbours) and a z extent one past two tiles."""
     from simulation import _bdlib as B
     rng = np.random.default_rng(1)
     Nx, Ny, Nz = shape
