@@ -104,23 +104,25 @@ class Dataset:
 
 
 def generate_coupled(n, n_photons=1_000_000, seed=0, shape=(192, 192, 192), epochs_per_run=150, snapshot_every=3,
-                     progress=None, device=0, max_windows=4, mc_photons=0):
+                     progress=None, device=0, max_windows=4, mc_photons=0, power=None, theta_deg=None,
+                     spot_dia=None):
     """Dataset of n windows taken from coupled runs of the layered-stack workload,
     i.e. the states the coupled loop hands the network: every `snapshot_every`-th
-    epoch of a plain-MC run (random stack seed, power 20-80 W, angle 0-30 deg, spot
-    0.35-0.65 mm), the tiled beam windows of that state (up to max_windows), their
-    features (transport optics with the plume), and as targets one fresh n_photons
-    MC over the whole domain."""
+    epoch of a plain-MC run (random stack seed, never seed 0; power 20-80 W, angle
+    0-30 deg, spot 0.35-0.65 mm unless fixed by power / theta_deg / spot_dia), the
+    tiled beam windows of that state (up to max_windows), their features
+    (transport optics with the plume), and as targets one fresh n_photons MC over
+    the whole domain."""
     from simulation.coupled_ablation import CoupledAblationEngine
     from simulation.workloads import layered_stack
     rng = np.random.default_rng(seed)
     ds = Dataset(n, device=device)
     i = 0
     while i < n:
-        P = float(rng.uniform(20.0, 80.0))
-        th = float(rng.uniform(0.0, 30.0))
-        spot = float(rng.uniform(0.35, 0.65))
-        kw = layered_stack(shape=shape, seed=int(rng.integers(1 << 30)), power=P, theta_deg=th, spot_dia=spot,
+        P = float(rng.uniform(20.0, 80.0)) if power is None else float(power)
+        th = float(rng.uniform(0.0, 30.0)) if theta_deg is None else float(theta_deg)
+        spot = float(rng.uniform(0.35, 0.65)) if spot_dia is None else float(spot_dia)
+        kw = layered_stack(shape=shape, seed=int(rng.integers(1, 1 << 30)), power=P, theta_deg=th, spot_dia=spot,
                            plume_grid=tuple(s // 2 for s in shape))
         eng = CoupledAblationEngine(**kw)
         try:

@@ -89,6 +89,11 @@ def main(argv=None):
     ap.add_argument("--coupled", type=int, default=0, help="windows taken from coupled runs (added to --samples)")
     ap.add_argument("--coupled-holdout", type=int, default=64)
     ap.add_argument("--np-len", type=int, default=80, help="transform length")
+    ap.add_argument("--coupled-shape", default="192,192,192", help="grid of the coupled runs")
+    ap.add_argument("--coupled-epochs", type=int, default=150, help="epochs per coupled run")
+    ap.add_argument("--coupled-power", type=float, default=None, help="fixed power [W] (default: 20-80 W)")
+    ap.add_argument("--coupled-theta", type=float, default=None, help="fixed angle [deg] (default: 0-30)")
+    ap.add_argument("--coupled-spot", type=float, default=None, help="fixed spot [mm] (default: 0.35-0.65)")
     ap.add_argument("--mc-photons", type=int, default=0,
                     help="train the refining network: channel 5 is the fluence of a run of this many photons")
     ap.add_argument("--out", required=True)
@@ -103,8 +108,11 @@ def main(argv=None):
     ds = data.generate(ntr, n_photons=a.photons, seed=a.seed, progress=1024, mc_photons=mp) if ntr else None
     ho = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000, mc_photons=mp)
     ho2 = data.generate(nho, n_photons=a.photons, seed=a.seed + 1000, mc_seed_offset=1, mc_photons=mp)
-    cds = data.generate_coupled(nco, n_photons=a.photons, seed=a.seed + 7, progress=512, mc_photons=mp) if nco else None
-    cho = data.generate_coupled(ncho, n_photons=a.photons, seed=a.seed + 2000, mc_photons=mp) if ncho else None
+    ckw = dict(n_photons=a.photons, mc_photons=mp, shape=tuple(int(v) for v in a.coupled_shape.split(",")),
+               epochs_per_run=a.coupled_epochs, power=a.coupled_power, theta_deg=a.coupled_theta,
+               spot_dia=a.coupled_spot)
+    cds = data.generate_coupled(nco, seed=a.seed + 7, progress=512, **ckw) if nco else None
+    cho = data.generate_coupled(ncho, seed=a.seed + 2000, **ckw) if ncho else None
     t_data = time.perf_counter() - t0
     print(f"data: {ntr} train + {nho} held-out samples (+ independent MC of the held-out) in {t_data:.1f} s",
           flush=True)
@@ -142,7 +150,8 @@ def main(argv=None):
                "epochs": a.epochs,
                "modes": list(net.modes), "np_len": net.np_len, "data_s": round(t_data, 1),
                "train_s": round(t_train, 1), "heldout": evaluate(net, ho, 0, nho, nb, other=ho2),
-               "coupled_windows": nco, "coupled_heldout": evaluate(net, cho, 0, ncho, nb) if cho else None}
+               "coupled_windows": nco, "coupled": {k: (list(v) if isinstance(v, tuple) else v)
+                                                    for k, v in ckw.items() if k not in ("n_photons", "mc_photons")}, "coupled_heldout": evaluate(net, cho, 0, ncho, nb) if cho else None}
     net.save(a.out, history=json.dumps(hist), summary=json.dumps(summary))
     print(json.dumps(summary), flush=True)
     net.close()
