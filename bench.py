@@ -265,9 +265,10 @@ def main():
                          "; surrogate on up to four 64^3 windows tiled along the beam" +
                          (f", its {args.correct_block}^3-block totals from a {args.mc_lo}-photon MC each epoch"
                           if args.mc_lo else "") +
-                         "; judge every 10th epoch: the epoch's MC and a second independent MC of the same state, "
-                         "fallback when the surrogate-vs-MC L2 exceeds the MC-vs-MC L2 by more than 5% in "
-                         "quadrature"),
+                         "; judge every 10th epoch against the epoch's MC, fallback when the surrogate-vs-MC L2 "
+                         "exceeds the MC-vs-MC L2 of two such runs by more than 5% in quadrature (MC-vs-MC " +
+                         ("from the low-photon run, scaled by photon count)" if args.mc_lo else
+                          "from a second independent run)")),
             "voxels": p["nvox"], "photons_per_epoch": p["n_photons"], "substeps_per_epoch": p["n_sub"],
             "mode": "plain MC" if args.plain_only else "surrogate+MC",
         },
@@ -312,8 +313,9 @@ def main():
                                "judged_rel_l2_block4": jb, "judged_rel_l2_voxel": s["judged_voxel"],
                                "mc_vs_mc_block4": s["mc_vs_mc_block4"], "excess_block4": s["excess_block4"],
                                "judge_passes": s["judge_passes"], "judgements": len(jb),
-                               "gate": "d_s^2 <= d_mm^2 + 0.05^2 (block-4 rel L2; d_mm: a second independent "
-                                       "MC of the same state)",
+                               "gate": "d_s^2 <= d_mm^2 + 0.05^2 (block-4 rel L2; d_mm: two independent "
+                                       "150k-photon MC runs of the state, " +
+                                       ("estimated from the low-photon run" if args.mc_lo else "measured") + ")",
                                "mc_anchored_scale": s["scale_after_judges"],
                                "max_judged_rel_l2_block4": max(jb) if jb else None, **s["state"]}
     if not args.no_cpu_baseline:

@@ -22,9 +22,12 @@ per-voxel figure is recorded beside it.
 Noise-aware gate (``noise_aware``, on by default).  Two independent MC runs of
 one state already differ, at the bench's 150k photons by 4-6 % (block-4) early
 and 15 % once the crater is 2.5 mm deep, so no field can be within 5 % of one
-MC run there, plain MC included.  Each judge epoch therefore also runs a
-second MC of the same state (same photon count, disjoint photon ids) and
-measures the MC-vs-MC distance d_mm beside the surrogate-vs-MC distance d_s.
+MC run there, plain MC included.  Each judge epoch therefore also measures
+the MC-vs-MC distance d_mm beside the surrogate-vs-MC distance d_s: from a
+second MC of the same state (same photon count, disjoint photon ids), or,
+when the low-photon run below exists, from that run's distance d_lo to the
+epoch's MC, d_mm^2 = 2 d_lo^2 / (N / N_lo + 1) (independent photons: the
+variance of each run is inversely proportional to its photon count).
 The prediction passes when d_s^2 <= d_mm^2 + tol^2: it agrees with the epoch's
 MC as well as a second MC run would, up to tol in quadrature.  Since
 E d_s^2 = e^2 + s^2 and E d_mm^2 = 2 s^2 (e: the prediction's distance from
@@ -68,6 +71,13 @@ def gate_passes(d_s, d_mm, tol):
     if d_mm is None:
         return d_s <= tol
     return d_s * d_s <= d_mm * d_mm + tol * tol
+
+
+def mc_vs_mc_from_lo(d_lo, n_photons, n_lo):
+    """The distance two independent n_photons runs would have, from the distance
+    d_lo of an independent n_lo run to one of them: E d_lo^2 = (n/n_lo + 1) s^2 and
+    E d_mm^2 = 2 s^2 with s^2 the variance of an n_photons run."""
+    return d_lo * (2.0 / (n_photons / n_lo + 1.0)) ** 0.5
 
 
 def excess(d_s, d_mm):
@@ -145,7 +155,7 @@ class SurrogateCoupledEngine:
             if compare:
                 beam = eng._epoch_beam()
                 self._predict(beam, into_q=False)
-                if self.noise_aware:
+                if self.noise_aware and not self.mc_lo:
                     # a second MC of the same state: the photon count of the epoch's own run, disjoint ids
                     eng._ctx.mc_run(noisy_beam(beam, beam.n_photons, JUDGE_PHOTON_OFFSET))
                     eng._ctx.sur_keep_aux()
@@ -156,7 +166,9 @@ class SurrogateCoupledEngine:
                 k = 1 if self.metric == "block4" else 0
                 e = err[k]
                 if self.noise_aware:
-                    mm = eng._ctx.sur_judge_aux()[k]
+                    mm = eng._ctx.sur_judge_aux()[k]   # the kept run against the epoch's MC
+                    if self.mc_lo:
+                        mm = mc_vs_mc_from_lo(mm, eng.n_photons, self.mc_lo)
                 self.fallback = not gate_passes(e, mm, self.tol)
                 self.judged = True
                 e0 = eng.epoch - 1   # the epoch just judged

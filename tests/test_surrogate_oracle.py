@@ -172,7 +172,7 @@ def test_noise_aware_gate_separates_noise_from_bias():
     Poisson-like noise: a candidate that is itself an independent MC run of the
     same noise passes (e = s), one 10 % off in amplitude fails, and the plain
     L2 test fails both once s exceeds the tolerance."""
-    from surrogate.coupled import excess, gate_passes
+    from surrogate.coupled import excess, gate_passes, mc_vs_mc_from_lo
     rng = np.random.default_rng(9)
     truth = rng.gamma(2.0, 1.0, 200_000)
 
@@ -192,6 +192,10 @@ def test_noise_aware_gate_separates_noise_from_bias():
         biased = 1.10 * truth
         assert not gate_passes(d(biased, m1), d_mm, 0.05)
         assert abs(excess(d(biased, m1), d_mm) - np.sqrt(0.10 ** 2 - sig ** 2)) < 0.02
+        # d_mm from an independent run with a quarter of the photons (twice the noise)
+        lo = mc(2.0 * sig)
+        est = mc_vs_mc_from_lo(d(lo, m1), 4.0, 1.0)
+        assert abs(est - np.sqrt(2) * sig) < 0.05 * np.sqrt(2) * sig
     # the plain test rejects an exact copy of the expected field when one run is noisy
     m1 = mc(0.08)
     assert not gate_passes(d(truth, m1), None, 0.05)
