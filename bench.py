@@ -101,20 +101,19 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_substep_traffic_layered_256x
 
 def measured_traffic(tuning):
     """HBM bytes per one-step sub-step launch on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None when
-    that summary was measured with another tile shape than this run's."""
-    tile, traffic = None, None
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated) for the tile shape
+    this run picked, or None when the summary has no line for that tile."""
+    want = f"{tuning['substep_tile_y']}x{tuning['substep_tile_x']}"
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("- tile:"):
-                    by, xc = line.split(":")[1].split("(")[0].split("x")
-                    tile = (int(by), int(xc))
-                elif line.startswith("- traffic_bytes_per_launch:"):
-                    traffic = int(float(line.split(":")[1]))
+                if line.startswith("- tile ") and "traffic_bytes_per_launch" in line:
+                    tile = line[len("- tile "):].split(":")[0].strip()
+                    if tile == want:
+                        return int(float(line.split("traffic_bytes_per_launch")[1]))
     except (OSError, ValueError):
         return None
-    return traffic if tile == (tuning["substep_tile_y"], tuning["substep_tile_x"]) else None
+    return None
 
 
 def run_surrogate(kw, steps, warmup, weights, mc_lo, correct_block):
