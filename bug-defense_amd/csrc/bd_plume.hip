@@ -72,6 +72,12 @@ int launch_plume_substeps(Ctx *c, int n, double dt, double decay)
     const bd_plume_params &p = c->plume;
     double dx, dy, dz;
     plume_spacing(c, dx, dy, dz);
+    if (p.dt_clamp) {
+        // the step the reference's Metal solver takes (metal_ablation_solver.py:234-236): stable for any
+        // grid, at the cost of diffusing less per sub-step than the nominal coefficients
+        const double d_rate = p.D_xy * (1.0 / (dx * dx) + 1.0 / (dy * dy)) + p.D_z / (dz * dz);
+        if (d_rate > 1e-6) dt = std::min(dt, 0.45 / (2.0 * d_rate));
+    }
     for (int s = 0; s < n; ++s) {
         hipLaunchKernelGGL(k_plume_diffuse, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->P2, c->pvoid,
                            p.nx, p.ny, p.nz, (float)(1.0 / (dx * dx)), (float)(1.0 / (dy * dy)),

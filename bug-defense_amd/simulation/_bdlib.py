@@ -62,6 +62,7 @@ class PlumeParams(ctypes.Structure):
         ("D_xy", ctypes.c_double), ("D_z", ctypes.c_double), ("v_drift", ctypes.c_double),
         ("tau_linger", ctypes.c_double), ("mu_max", ctypes.c_double), ("mu_a_max", ctypes.c_double),
         ("mu_s_max", ctypes.c_double), ("delta_mu", ctypes.c_double), ("blast_expansion", ctypes.c_int32),
+        ("dt_clamp", ctypes.c_int32),
     ]
 
 

@@ -256,6 +256,7 @@ class CoupledAblationEngine:
                 blast_sigma_xy_mm=float(pc.get('blast_sigma_xy_mm', 0.080)),
                 blast_sigma_z_mm=float(pc.get('blast_sigma_z_mm', 0.090)),
                 blast_z_bias_mm=float(pc.get('blast_z_bias_mm', -0.050)),
+                dt_clamp=bool(pc.get('dt_clamp', False)),
                 _context=ctx, _grid_shape=tuple(plume_grid) if plume_grid is not None else shape)
         self.metal_ablation_ctx = None
         self._ds_arrays = None

@@ -30,6 +30,7 @@ def _plume_params(pl, enabled=1) -> B.PlumeParams:
     p.mu_max, p.mu_a_max, p.mu_s_max = pl.mu_max, pl.mu_a_max, pl.mu_s_max
     p.delta_mu = pl.delta_mu
     p.blast_expansion = 1 if pl.blast_expansion else 0
+    p.dt_clamp = 1 if getattr(pl, 'dt_clamp', False) else 0
     return p
 
 
@@ -50,7 +51,7 @@ class MultispectralPlumeField:
                  sigma_xy_mm=0.060, sigma_z_mm=0.040, z_drift_mm=0.030, delta_mu_per_voxel=None, mu_a_max=None,
                  sigma_mm=None, model='separable', D_plume=2.0, D_xy=None, D_z=None, v_drift=80.0,
                  blast_expansion=True, blast_sigma_xy_mm=0.080, blast_sigma_z_mm=0.090, blast_z_bias_mm=-0.050,
-                 _context=None, _grid_shape=None):
+                 dt_clamp=False, _context=None, _grid_shape=None):
         self.shape = tuple(int(v) for v in shape)
         self.dx, self.dy, self.dz = float(dx_mm), float(dy_mm), float(dz_mm)
         self.tau_linger = float(tau_linger_s)
@@ -61,6 +62,7 @@ class MultispectralPlumeField:
         self.D_z = float(D_z) if D_z is not None else self.D_plume
         self.v_drift = float(v_drift)
         self.blast_expansion = bool(blast_expansion)
+        self.dt_clamp = bool(dt_clamp)   # extra keyword: the Metal solver's diffusion step (see bd_plume_params)
         self.blast_sigma_xy = float(blast_sigma_xy_mm)
         self.blast_sigma_z = float(blast_sigma_z_mm)
         self.blast_z_bias = float(blast_z_bias_mm)

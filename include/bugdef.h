@@ -99,6 +99,9 @@ typedef struct {
     double D_xy, D_z, v_drift, tau_linger;
     double mu_max, mu_a_max, mu_s_max, delta_mu;
     int32_t blast_expansion;
+    int32_t dt_clamp;              /* 1: diffusion step min(dt, 0.45 / (2 (D_xy (1/dx^2 + 1/dy^2) + D_z/dz^2))),
+                                      the step of the reference's Metal solver (metal_ablation_solver.py:234-236);
+                                      0: dt, the CPU scheme (plume.py:18-80) */
 } bd_plume_params;
 
 /* Beam and run controls for one Monte Carlo call (voxel_mc_3d.py:425-445). */

@@ -608,6 +608,8 @@ class OracleEngine:
                 blast_sigma_xy_mm=float(pc.get('blast_sigma_xy_mm', 0.080)),
                 blast_sigma_z_mm=float(pc.get('blast_sigma_z_mm', 0.090)),
                 blast_z_bias_mm=float(pc.get('blast_z_bias_mm', -0.050)))
+            # the Metal solver's diffusion step (metal_ablation_solver.py:234-236), when asked for
+            self.plume_dt_clamp = bool(pc.get('dt_clamp', False))
         self._to_remove = np.zeros(shape, dtype=np.bool_)
         self._init_ds_cs_arrays()
         self.t = 0.0
@@ -740,6 +742,12 @@ class OracleEngine:
             P, Pn = self.plume.raw_field, self.plume._tmp_field
             tl = self.plume.tau_linger
             decay = float(np.exp(-self.dt_sub / tl)) if (tl > 0.0 and not np.isinf(tl)) else 1.0
+            dt_p = self.dt_sub
+            if getattr(self, 'plume_dt_clamp', False):
+                pl = self.plume
+                d_rate = pl.D_xy * (1.0 / pl.dx ** 2 + 1.0 / pl.dy ** 2) + pl.D_z / pl.dz ** 2
+                if d_rate > 1e-6:
+                    dt_p = min(dt_p, 0.45 / (2.0 * d_rate))
         N = self.Nx, self.Ny, self.Nz
         for _ in range(self.n_substeps):
             thermal_step(T, Tn, Q_th, self.k_grid, self.rho_cp_grid, *N, self.dx, self.dy, self.dz, self.dt_sub,
@@ -750,7 +758,7 @@ class OracleEngine:
             if self._arrhenius_char is not None:
                 char(Tn, self.chi, dr8, ab8, self._arrhenius_char[0], self._arrhenius_char[1], self.dt_sub)
             if has_diff:
-                plume_diffuse(P, Pn, v8, *self.pshape, self.plume.dx, self.plume.dy, self.plume.dz, self.dt_sub,
+                plume_diffuse(P, Pn, v8, *self.pshape, self.plume.dx, self.plume.dy, self.plume.dz, dt_p,
                               self.plume.D_xy, self.plume.D_z, self.plume.v_drift, decay)
                 P, Pn = Pn, P
             T, Tn = Tn, T

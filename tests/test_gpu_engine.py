@@ -178,9 +178,11 @@ def test_ablation_extensions_reference_fixtures():
         assert e.step_epoch()['absorbed_frac'] > 0.0
 
 
-def test_engine_against_oracle_engine(oracle):
+@pytest.mark.parametrize("dt_clamp", [False, True])
+def test_engine_against_oracle_engine(oracle, dt_clamp):
     """Coupled epochs with dry state, char, air gap and diffusion plume: the device engine and
-    the FP64 oracle engine trace the same photon streams; compare the state after 6 epochs."""
+    the FP64 oracle engine trace the same photon streams; compare the state after 6 epochs.
+    With dt_clamp the plume takes the Metal solver's diffusion step on both sides."""
     from simulation.coupled_ablation import CoupledAblationEngine
     from simulation.run_multispectral_matrix import CHAR_STATE
     shape = (32, 32, 40)
@@ -207,7 +209,7 @@ def test_engine_against_oracle_engine(oracle):
                   power=8.0, spot_dia=0.2, t_abl=100.0, t_ambient=20.0, dt_epoch=0.001, dt_sub=5e-5,
                   n_photons=40000, h_eff=15e-6, theta_inc_deg=22.0, char_state=CHAR_STATE,
                   plume_closure={'model': 'diffusion', 'D_xy': 1.0, 'D_z': 2.0, 'v_drift': 0.0,
-                                 'tau_linger_s': float('inf'), 'blast_expansion': False},
+                                 'tau_linger_s': float('inf'), 'blast_expansion': False, 'dt_clamp': dt_clamp},
                   initial_dry_mask=dry, initial_ablated_mask=abl, air_gap_range=(7, 10), wavelength_nm=532, seed=42)
     eng = CoupledAblationEngine(dry_state={kk: (v.copy() if isinstance(v, np.ndarray) else v)
                                            for kk, v in ds.items()}, **common)
@@ -228,6 +230,11 @@ def test_engine_against_oracle_engine(oracle):
     # removed-volume and char agree statistically
     assert abs(int(eng.ablated.sum()) - int(ref.ablated.sum())) <= 0.2 * max(10, int(ref.ablated.sum() - abl.sum()))
     assert eng.T_peak.max() >= float(Tg.max()) - 1e-4
+    # the plume masses agree statistically (same photon streams, so nearly the same removals)
+    pg = np.asarray(eng.plume.raw_field, dtype=np.float64)
+    pr = np.asarray(ref.plume.raw_field, dtype=np.float64)
+    assert pg.max() > 0.0
+    assert abs(pg.sum() - pr.sum()) <= 0.2 * pr.sum()
 
 
 def test_graph_epochs_match_eager_epochs():

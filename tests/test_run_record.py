@@ -72,3 +72,26 @@ def test_recorder_matches_host_slices(tmp_path):
     rec.save(p, wavelength_nm=532.0, power_w=40.0)
     k = analyze_npz(p, pulse_duration_s=0.012)
     assert k['final_depth_um'] >= 0.0 and np.isfinite(k['core_max_T'])
+
+
+def test_beetle_model_layout():
+    """run_highres_knowledge_suite.py:126-185: 21 slices of ambient air (ablated from
+    t = 0), a 165 um dry dorsal shell, the 300 um subelytral air gap on the air
+    properties, then the wet body and a 90 um dry ventral shell; optics from
+    eval_single_layer at the wavelength; the tilted beam enters the dorsal
+    surface at x = 1 mm and reaches x = 2 mm at the ventral face."""
+    import math
+    from simulation import beetle_suite as S
+    from simulation.tissue_stack import BASE_LAYERS, eval_single_layer
+    g = S.build_beetle_model(532)
+    col = lambda a: a[100, 100, :]
+    assert col(g['initial_ablated'])[:21].all() and not col(g['initial_ablated'])[21:].any()
+    dry = col(g['dry_mask'])
+    assert dry[21:32].all() and not dry[32:178].any() and dry[178:].all()
+    k = col(g['k_grid'])
+    assert (k[32:52] == 0.026e-3).all() and k[52] == 0.48e-3 and k[90] == 0.50e-3
+    ev = eval_single_layer(BASE_LAYERS[6], 532.0)   # hemolymph
+    assert col(g['optical']['mu_a'])[73] == float(ev['mu_a']) and col(g['optical']['n'])[89] == float(ev['n'])
+    assert abs(S.X_TOP + S.TAN_THETA * S.L_AIR_TOP - 1.0) < 1e-12
+    assert abs(S.X_TOP + S.TAN_THETA * S.LZ - 2.0) < 1e-12
+    assert abs(S.THETA_INC_DEG - math.degrees(math.atan(1.0 / 2.445))) < 1e-9
