@@ -1,9 +1,11 @@
 """Extract the reference's recorded campaign KPIs into a small fixture.
 
 Reads ``knowledge/beetle/kpi_cache.json`` from the reference checkout: the KPIs
-its own 200x200x184 beetle runs produced (run_highres_knowledge_suite.py, the
-single-wavelength runs ``matrix_<wl>nm_<P>w.npz``, reduced by
-compute_suite_metrics.py), and keeps the run outcomes of those 21 runs.  Run
+its own 200x200x184 beetle runs produced, reduced by compute_suite_metrics.py:
+the 21 single-wavelength runs of run_highres_knowledge_suite.py
+(``matrix_<wl>nm_<P>w.npz``), and the dual-wavelength and no-plume runs of
+run_dual_permutation_suite.py (``matrix_dual_<wl1>nm_<wl2>nm.npz``,
+``matrix_<wl>nm_40w_no_plume.npz``).  Run
 once in the build container:
 
     python tests/golden/make_kpi_fixture.py /root/reference
@@ -25,12 +27,23 @@ def main(ref_root):
         d = json.load(f)
     out = {}
     for name, rec in sorted(d.items()):
-        m = re.fullmatch(r"matrix_(\d+)nm_(\d+)w\.npz", name)
-        if not m:
-            continue
         k = rec["kpis"]
-        out[f"{m.group(1)}nm_{m.group(2)}w"] = {"wavelength_nm": int(m.group(1)), "power_w": float(m.group(2)),
-                                               **{key: k[key] for key in KEYS}}
+        m = re.fullmatch(r"matrix_(\d+)nm_(\d+)w\.npz", name)
+        if m:   # run_highres_knowledge_suite.py runs
+            out[f"{m.group(1)}nm_{m.group(2)}w"] = {"kind": "single", "wavelength_nm": int(m.group(1)),
+                                                   "power_w": float(m.group(2)), **{key: k[key] for key in KEYS}}
+            continue
+        m = re.fullmatch(r"matrix_dual_(\d+)nm_(\d+)nm\.npz", name)
+        if m:   # run_dual_permutation_suite.py, two wavelengths
+            out[f"dual_{m.group(1)}nm_{m.group(2)}nm"] = {"kind": "dual", "wavelength_nm": int(m.group(1)),
+                                                         "wavelength2_nm": int(m.group(2)),
+                                                         **{key: k[key] for key in KEYS}}
+            continue
+        m = re.fullmatch(r"matrix_(\d+)nm_(\d+)w_no_plume\.npz", name)
+        if m:   # run_dual_permutation_suite.py --no-plume
+            out[f"{m.group(1)}nm_{m.group(2)}w_no_plume"] = {"kind": "no_plume", "wavelength_nm": int(m.group(1)),
+                                                            "power_w": float(m.group(2)),
+                                                            **{key: k[key] for key in KEYS}}
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kpi_highres_reference.json")
     with open(dst, "w") as f:
         json.dump({"source": "knowledge/beetle/kpi_cache.json (reference checkout)", "runs": out}, f, indent=1,

@@ -1,5 +1,5 @@
-"""Results parity with the reference's recorded campaign: two of its 200x200x184
-beetle runs (run_highres_knowledge_suite.py) on this engine, reduced to the KPIs
+"""Results parity with the reference's recorded campaign: four of its 200x200x184
+beetle runs (run_highres_knowledge_suite.py, run_dual_permutation_suite.py) on this engine, reduced to the KPIs
 by compute_suite_metrics' restatement, against the values the reference's own
 runs produced (tests/golden/kpi_highres_reference.json, from its kpi_cache.json).
 
@@ -21,11 +21,20 @@ REL = {"final_depth_um": 0.05, "avg_depth_um": 0.05, "avg_hole_dia_mm": 0.06, "v
        "v_boil_mm3": 0.06, "core_avg_T": 0.10, "ven_avg_T": 0.10, "min_trans": 0.03}
 
 
-@pytest.mark.parametrize("run", ["532nm_40w", "10600nm_20w"])
+@pytest.mark.parametrize("run", ["532nm_40w", "10600nm_20w", "dual_10600nm_532nm", "532nm_40w_no_plume"])
 def test_campaign_run_matches_reference_kpis(tmp_path, run):
-    from simulation.beetle_suite import run_single_highres
+    """A single-wavelength run, a dual-wavelength run (the mid-run switch of the
+    intact voxels' optics, run_dual_permutation_suite.py:249-279) and a run with
+    the plume disabled."""
+    from simulation import beetle_suite as S
     ref = json.load(open(os.path.join(HERE, "golden", "kpi_highres_reference.json")))["runs"][run]
-    k = run_single_highres(ref["wavelength_nm"], ref["power_w"], str(tmp_path / f"matrix_{run}.npz"), seed=1)
+    path = str(tmp_path / f"matrix_{run}.npz")
+    if ref["kind"] == "dual":
+        k = S.run_dual(ref["wavelength_nm"], ref["wavelength2_nm"], path, seed=1)
+    elif ref["kind"] == "no_plume":
+        k = S.run_no_plume(ref["wavelength_nm"], path, seed=1)
+    else:
+        k = S.run_single_highres(ref["wavelength_nm"], ref["power_w"], path, seed=1)
     print(run, {key: (round(k[key], 4), round(ref[key], 4)) for key in REL})
     for key, tol in REL.items():
         assert abs(k[key] - ref[key]) <= tol * abs(ref[key]), (key, k[key], ref[key])
