@@ -307,3 +307,69 @@ def test_mid_run_wavelength_switch_matches_oracle(oracle):
     rise = float(Tr.max() - 20.0)
     assert abs(float(Tg.max()) - float(Tr.max())) < 0.05 * rise
     assert eng.plume.wavelength == 1470.0 and eng.wavelength_nm == 1470.0
+
+
+def _sagi_run():
+    """knowledge/simulation_lit_comparison/code/sagi_stages.py:26-55 (run_sagi_stages) on this
+    engine: a quasi-1-D wet absorber (McMillan et al. tissue constants, mcmillan_benchmark.py:32-41)
+    under a beam ten times wider than the 0.2 mm column, boiled dry and removed at 300 C."""
+    from simulation.coupled_ablation import CoupledAblationEngine
+    W0, PRO0 = 0.75, 0.25
+    rho_wet = 1000.0 / (W0 + 0.649 * PRO0)
+    cp_wet = 1000.0 * (4.2 * W0 + 1.09 * PRO0)
+    k_wet = rho_wet * (6.28e-4 * W0 + 1.17e-4 * PRO0)
+    rho_dry, cp_dry = 1000.0 / 0.649, 1000.0 * 1.09
+    k_dry = rho_dry * 1.17e-4
+    mu_a_wet, h_lat = 85.0, W0 * rho_wet * 2.26e6 * 1e-9
+    I, Ta, t_end, dz, Nz, dt_epoch, n_photons = 0.5, 300.0, 1.2, 0.005, 100, 0.002, 60000
+    Nx = Ny = 4
+    Lx = 0.2
+    w0 = 10.0 * (Lx / 2.0)
+    shp = (Nx, Ny, Nz)
+    eng = CoupledAblationEngine(
+        optical_props={'mu_a': np.full(shp, mu_a_wet), 'mu_s': np.zeros(shp), 'g': np.zeros(shp), 'n': np.ones(shp)},
+        k_grid=np.full(shp, k_wet * 1e-3), rho_cp_grid=np.full(shp, rho_wet * cp_wet * 1e-9),
+        h_lat_grid=np.full(shp, h_lat), Lx=Lx, Ly=Lx, Lz=Nz * dz,
+        power=I * np.pi * w0 ** 2 / 2.0, spot_dia=2 * w0, t_abl=100.0, t_ambient=25.0,
+        dt_epoch=dt_epoch, dt_sub=4e-5, n_photons=n_photons, h_eff=0.0, seed=3,
+        dry_state=dict(mu_a=0.25 * mu_a_wet, mu_s=0.0, g=0.0, n=1.0, k=k_dry * 1e-3,
+                       rho_cp=rho_dry * cp_dry * 1e-9, t_remove=Ta))
+    rec = {k: [] for k in ('t_s', 'T_surface', 'depth_mm', 'n_dry_layers')}
+    try:
+        for _ in range(int(round(t_end / dt_epoch))):
+            eng.step_epoch()
+            col_abl = np.asarray(eng.ablated).all(axis=(0, 1))
+            k_surf = int(np.argmin(col_abl)) if not col_abl.all() else Nz - 1
+            rec['t_s'].append(eng.t)
+            rec['T_surface'].append(float(np.asarray(eng.T)[:, :, k_surf].mean()))
+            rec['depth_mm'].append(float(col_abl.sum() * dz))
+            rec['n_dry_layers'].append(int(np.asarray(eng.dry).any(axis=(0, 1)).sum()))
+    finally:
+        eng.close()
+    return {k: np.asarray(v) for k, v in rec.items()}
+
+
+def test_staged_ablation_series_matches_reference_record():
+    """The reference engine's recorded staged-ablation series (sagi_stages.json, 600 epochs):
+    heating to boiling, the 100 C plateau, drying, dry heating to 300 C, removal.  The stage
+    times, the surface temperature and the crater depth follow the record."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "sagi_stages_reference.json")))
+    r = _sagi_run()
+    t = np.asarray(ref['t_s'])
+    assert np.allclose(r['t_s'], t, atol=1e-9)
+    Tr, dr, ndr = (np.asarray(ref[k]) for k in ('T_surface', 'depth_mm', 'n_dry_layers'))
+    def first(mask):
+        return float(t[np.argmax(mask)]) if mask.any() else float('inf')
+    for name, a, b in (("boiling", r['T_surface'] >= 99.9, Tr >= 99.9),
+                       ("first dry layer", r['n_dry_layers'] > 0, ndr > 0),
+                       ("first removal", r['depth_mm'] > 0, dr > 0)):
+        print(name, first(a), first(b))
+        assert abs(first(a) - first(b)) <= 0.1 * first(b) + 0.004, name   # the FP64 oracle engine: 0, 8, 2 ms
+    dT = np.abs(r['T_surface'] - Tr)
+    print("surface T: max |dT|", float(dT.max()), "mean", float(dT.mean()))
+    assert float(np.median(dT)) < 1.0 and float(dT.mean()) < 3.0
+    dd = np.abs(r['depth_mm'] - dr)
+    print("depth: max |d|", float(dd.max()), "at end", float(r['depth_mm'][-1]), float(dr[-1]))
+    assert float(dd.max()) <= 0.0101 and abs(r['depth_mm'][-1] - dr[-1]) <= 0.0051
