@@ -373,3 +373,50 @@ def test_staged_ablation_series_matches_reference_record():
     dd = np.abs(r['depth_mm'] - dr)
     print("depth: max |d|", float(dd.max()), "at end", float(r['depth_mm'][-1]), float(dr[-1]))
     assert float(dd.max()) <= 0.0101 and abs(r['depth_mm'][-1] - dr[-1]) <= 0.0051
+
+
+def test_mcmillan_pixel_depths_match_reference_engine():
+    """The reference engine's crater depths of the McMillan et al. (2021) pixel runs
+    (mcmillan_benchmark.py:46-99: an 81 x 81 x 160 grid of 7.5 um, a pure absorber of
+    850 /cm, dry tissue at 25 % of the wet absorption, removal at 500 C, triangular
+    pulses of 50-400 mJ at 1.2 ms per mJ, 20 000 photons per 2 ms epoch), recorded as
+    the model values of benchmark_results.json (M-<E>-exp)."""
+    import json
+    import os
+    from simulation.coupled_ablation import CoupledAblationEngine
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_engine_values.json")))["values"]
+    W0, PRO0 = 0.75, 0.25
+    rho_wet = 1000.0 / (W0 + 0.649 * PRO0)
+    cp_wet = 1000.0 * (4.2 * W0 + 1.09 * PRO0)
+    k_wet = rho_wet * (6.28e-4 * W0 + 1.17e-4 * PRO0)
+    rho_dry, cp_dry = 1000.0 / 0.649, 1000.0 * 1.09
+    k_dry = rho_dry * 1.17e-4
+    mu_a_wet, h_lat = 85.0, W0 * rho_wet * 2.26e6 * 1e-9
+    N, voxel, dt = 81, 0.0075, 0.002
+    Nz = int(round(1.2 / voxel))
+    shp, L = (N, N, Nz), N * voxel
+    got = {}
+    for E in (50, 100, 150, 200, 250, 300, 350, 400):
+        n_ep = int(round(1.2e-3 * E / dt))
+        t = (np.arange(n_ep) + 0.5) * dt
+        p = np.clip(1.0 - np.abs(2.0 * t / (1.2e-3 * E) - 1.0), 0.0, None)
+        P = p * (E * 1e-3) / (p.sum() * dt)
+        eng = CoupledAblationEngine(
+            optical_props={'mu_a': np.full(shp, mu_a_wet), 'mu_s': np.zeros(shp), 'g': np.zeros(shp),
+                           'n': np.ones(shp)},
+            k_grid=np.full(shp, k_wet * 1e-3), rho_cp_grid=np.full(shp, rho_wet * cp_wet * 1e-9),
+            h_lat_grid=np.full(shp, h_lat), Lx=L, Ly=L, Lz=Nz * voxel, power=1.0, spot_dia=0.25, t_abl=100.0,
+            t_ambient=5.0, dt_epoch=dt, dt_sub=4e-5, n_photons=20000, h_eff=0.0, beam_profile='tophat', seed=E,
+            dry_state=dict(mu_a=0.25 * mu_a_wet, mu_s=0.0, g=0.0, n=1.0, k=k_dry * 1e-3,
+                           rho_cp=rho_dry * cp_dry * 1e-9, t_remove=500.0))
+        try:
+            for i in range(n_ep):
+                eng.power = float(P[i])
+                eng.step_epoch()
+            got[E] = eng.centerline_depth_mm(N // 2, N // 2)
+        finally:
+            eng.close()
+    print({E: (round(got[E], 4), ref[f"M-{E}-exp"]["model"]) for E in got})
+    for E, d in got.items():
+        # within two voxels of the reference engine's depth
+        assert abs(d - ref[f"M-{E}-exp"]["model"]) <= 2 * voxel + 1e-9, E
