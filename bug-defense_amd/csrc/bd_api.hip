@@ -202,6 +202,8 @@ int bd_set_physics(bd_ctx *h, const bd_physics *p)
     BD_CHECK(p->dt_sub > 0.0, "bd_set_physics: dt_sub must be positive");
     BD_CHECK(p->air_gap_k0 < 0 || (p->air_gap_k0 <= p->air_gap_k1 && p->air_gap_k1 <= h->c.g.nz),
              "bd_set_physics: air_gap_range outside the grid");
+    // the captured epoch graph holds these values as kernel arguments
+    if (!h->c.has_phys || std::memcmp(&h->c.phys, p, sizeof(*p)) != 0) h->c.graph_stale = true;
     h->c.phys = *p;
     h->c.has_phys = true;
     return BD_OK;
@@ -211,6 +213,7 @@ int bd_set_plume(bd_ctx *h, const bd_plume_params *p)
 {
     BD_CHECK(h && p, "bd_set_plume: null argument");
     Ctx *c = &h->c;
+    if (std::memcmp(&c->plume, p, sizeof(*p)) != 0) c->graph_stale = true;
     if (!p->enabled) {
         c->plume = *p;
         return BD_OK;
@@ -251,6 +254,7 @@ int bd_set_plume_kernels(bd_ctx *h, int which, const float *kx, int nkx, const f
             BD_HIP(hipFree(dst[a]));
             dst[a] = nullptr;
         }
+        if (nk[a] != n[a] || r[a] != rr[a]) c->graph_stale = true;
         if (!dst[a]) BD_HIP(hipMalloc((void **)&dst[a], sizeof(float) * n[a]));
         BD_HIP(hipMemcpyAsync(dst[a], src[a], sizeof(float) * n[a], hipMemcpyHostToDevice, c->stream));
         nk[a] = n[a];
@@ -276,7 +280,10 @@ int bd_upload(bd_ctx *h, int field, int dtype, const void *host, int64_t n)
             BD_HIP(hipFree(f.ptr));
             f.ptr = nullptr;
         }
-        if (!f.ptr) BD_HIP(hipMalloc(&f.ptr, esz * (size_t)n));
+        if (!f.ptr) {
+            BD_HIP(hipMalloc(&f.ptr, esz * (size_t)n));
+            c->graph_stale = true;  // a new pointer (and per-voxel/scalar mode) for the captured kernels
+        }
         f.n = n;
         f.bytes = esz * (size_t)n;
         BD_TRY(stage_in(c, host, dsize(dtype) * (size_t)n));

@@ -276,7 +276,7 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
     BD_TRY(prepare_enqueued_epoch(c, n_substeps, "bd_epoch_graph"));
     const bool recapture = !c->graph_exec || c->graph_nsub != n_substeps || !same_beam(c->graph_beam, *beam) ||
                            c->graph_T != c->T || c->graph_P != c->P || c->graph_H != c->H ||
-                           c->graph_F != c->flags;
+                           c->graph_F != c->flags || c->graph_stale;
     if (recapture) {
         // allocate every lazily created buffer and tune the tiles before capture
         BD_TRY(launch_substeps(c, 0, 1, BD_SUB_PLATEAU, true));
@@ -318,6 +318,7 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
         BD_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
         c->graph_beam = *beam;
         c->graph_nsub = n_substeps;
+        c->graph_stale = false;
     }
     // per-epoch scalars -> device (outside the graph), replay; the readback is bd_epoch_graph_finish
     const double dt_epoch = n_substeps * c->phys.dt_sub;

@@ -119,6 +119,7 @@ struct Ctx {
     const float *graph_H = nullptr;
     const uint8_t *graph_F = nullptr;
     bool graph_pending = false;          // launched by bd_epoch_graph_launch, not yet finished
+    bool graph_stale = false;            // physics/plume values or parameter buffers changed since capture
     bd_beam pending_beam{};
     double pending_dt_epoch = 0.0;
 };

@@ -258,6 +258,28 @@ def test_graph_epochs_match_eager_epochs():
     assert e1.t == e2.t and e1.epoch == e2.epoch == 8
 
 
+def test_graph_epochs_follow_physics_changes():
+    """A captured epoch graph holds the physics as kernel arguments: a change of the
+    ablation threshold, the surface loss and the dry/char parameters between epochs
+    must reach the replayed graph exactly as it reaches the eager epoch."""
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.workloads import layered_stack
+    engines = [CoupledAblationEngine(graph=g, **layered_stack(shape=(40, 40, 48), n_photons=20000,
+                                                              plume_grid=(20, 20, 24), power=60.0))
+               for g in (False, True)]
+    for ep in range(6):
+        if ep == 3:
+            for e in engines:
+                e.t_abl = e.t_abl - 40.0
+                e.h_eff = 4.0 * e.h_eff + 50.0
+        d1, d2 = (e.step_epoch() for e in engines)
+        assert d1['absorbed_frac'] == pytest.approx(d2['absorbed_frac'], abs=1e-6)
+        assert abs(d1['n_new'] - d2['n_new']) <= max(2, 0.02 * d1['n_new'])
+    T1, T2 = (np.asarray(e.T) for e in engines)
+    assert float(np.max(np.abs(T1 - T2))) < 1e-2 * float(T1.max() - 20.0) + 1e-3
+    assert int(engines[0].ablated.sum()) == pytest.approx(int(engines[1].ablated.sum()), rel=0.02, abs=2)
+
+
 def _switch(engine, geo2, wl2, p2):
     """The reference's mid-run wavelength switch, verbatim in effect
     (run_dual_permutation_suite.py:257-283): intact-voxel optics and dry-state
