@@ -318,16 +318,6 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
             else if (sy <= sz) { axis = 1; s_exit = sy; }
             else { axis = 2; s_exit = sz; }
             s_exit = fmaxf(s_exit, 0.f);
-            // optics of the exit voxel, loaded before the interaction test so that the
-            // load's latency overlaps the rest of the event (unused if the photon
-            // interacts inside the voxel)
-            int qx = p.ix, qy = p.iy, qz = p.iz;
-            if (axis == 0) qx += p.ux > 0.f ? 1 : -1;
-            else if (axis == 1) qy += p.uy > 0.f ? 1 : -1;
-            else qz += p.uz > 0.f ? 1 : -1;
-            float4 o_next = make_float4(0.f, 0.f, 0.f, 1.f);
-            if (qx >= 0 && qx < Nx && qy >= 0 && qy < Ny && qz >= 0 && qz < Nz)
-                o_next = a.opt[((int64_t)qx * Ny + qy) * Nz + qz];
             t_ev += 1;
             p.ev++;
             if (mut > 1e-12f && mut * s_exit >= p.tau) {
@@ -394,6 +384,17 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                 p.tau = -__logf(fmaxf(rng_u(rng), 1e-12f));
             } else {
                 // ---- full chord to the exit face (voxel_mc_3d.py:267-337)
+                // Optics of the exit voxel, loaded first so that the load's latency overlaps
+                // the rest of the chord.  Only this branch loads: on gfx9 loads and the tally
+                // atomics share one in-order counter, so a load left in flight on the
+                // interaction branch would make that branch wait for it and for its atomics.
+                int qx = p.ix, qy = p.iy, qz = p.iz;
+                if (axis == 0) qx += p.ux > 0.f ? 1 : -1;
+                else if (axis == 1) qy += p.uy > 0.f ? 1 : -1;
+                else qz += p.uz > 0.f ? 1 : -1;
+                float4 o_next = make_float4(0.f, 0.f, 0.f, 1.f);
+                if (qx >= 0 && qx < Nx && qy >= 0 && qy < Ny && qz >= 0 && qz < Nz)
+                    o_next = a.opt[((int64_t)qx * Ny + qy) * Nz + qz];
                 p.tau -= mut * s_exit;
                 tally(v, 0.f, p.w * s_exit);
                 p.x += s_exit * p.ux;

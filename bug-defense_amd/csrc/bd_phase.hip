@@ -12,6 +12,8 @@
 // of the same stored fields.
 #include "bd_internal.hpp"
 
+#include <cmath>
+
 namespace bd {
 
 struct PParam {
@@ -265,5 +267,46 @@ int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J)
     return BD_OK;
 }
 
+// validation_benchmarks.py:116-121 (the DOP option of the scan benchmark): before
+// each epoch, every intact voxel's reduced scattering follows its Arrhenius damage,
+// alpha = 1 - exp(-Omega), mu_s' = (1 - alpha) mu_s'_native + alpha mu_s'_coag, and
+// mu_s = mu_s' / (1 - g).  FP64 arithmetic on the stored Omega, the FP32 result
+// stored, as the host formula in float64 gives when its result is uploaded.
+__global__ void k_damage_optics(float *__restrict__ mu_s, const float *__restrict__ Om,
+                                const uint8_t *__restrict__ flags, double musp_native, double musp_coag,
+                                double one_minus_g, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (flags[i] & F_ABL) continue;
+        const double alpha = 1.0 - exp(-(double)Om[i]);
+        const double musp = (1.0 - alpha) * musp_native + alpha * musp_coag;
+        mu_s[i] = (float)(musp / one_minus_g);
+    }
+}
+
+int launch_damage_optics(Ctx *c, double musp_native, double musp_coag, double g)
+{
+    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    hipLaunchKernelGGL(k_damage_optics, dim3(nb), dim3(256), 0, c->stream, c->mu_s, c->Om, c->flags, musp_native,
+                       musp_coag, 1.0 - g, c->nvox);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
 
 }  // namespace bd
+
+extern "C" int bd_damage_optics(bd_ctx *h, double musp_native, double musp_coag, double g)
+{
+    using namespace bd;
+    BD_CHECK(h, "bd_damage_optics: null argument");
+    BD_CHECK(std::isfinite(musp_native) && std::isfinite(musp_coag) && std::isfinite(g) && g < 1.0,
+             "bd_damage_optics: parameters must be finite with g < 1");
+    Ctx *c = &h->c;
+    if (!c->mu_s || !c->Om || !c->flags) {
+        set_error("bd_damage_optics: mu_s, Omega and the masks must be on the device first");
+        return BD_ESTATE;
+    }
+    BD_TRY(launch_damage_optics(c, musp_native, musp_coag, g));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    return BD_OK;
+}

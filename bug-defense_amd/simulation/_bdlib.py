@@ -144,6 +144,7 @@ _SIGS = {
     "bd_reduce": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                   ctypes.c_int),
     "bd_phase_map": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_damage_optics": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "bd_phase_slice": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                         ctypes.c_int64], ctypes.c_int),
     "bd_sync": ([ctypes.c_void_p], ctypes.c_int),
@@ -574,6 +575,10 @@ class Context:
         check(lib().bd_phase_map(self.handle, None if gap is None else _ptr(gap), _ptr(out), out.size),
               "bd_phase_map")
         return out
+
+    def damage_optics(self, musp_native, musp_coag, g):
+        """Intact voxels' mu_s from their Arrhenius damage (validation_benchmarks.py:116-121)."""
+        check(lib().bd_damage_optics(self.handle, float(musp_native), float(musp_coag), float(g)), "bd_damage_optics")
 
     def phase_slice(self, plane, index, gap_z=None, stride=1):
         """compute_phase_map on one plane, air gap as a per-z profile (uint8 [Nz])."""

@@ -416,6 +416,15 @@ class CoupledAblationEngine:
     def _invalidate(self):
         self.__dict__['_cache'].clear()
 
+    def apply_damage_optics(self, musp_native, musp_coag, g):
+        """The scan benchmark's DOP step (validation_benchmarks.py:116-121) on the device:
+        every intact voxel's mu_s becomes ((1 - alpha) musp_native + alpha musp_coag) / (1 - g)
+        with alpha = 1 - exp(-Omega).  Same result as the host loop body
+        ``eng.optical['mu_s'][intact] = musp[intact] / (1 - g)``, without moving the grids."""
+        self._flush_host_writes()
+        self._ctx.damage_optics(musp_native, musp_coag, g)
+        self.__dict__['_cache'].pop('optical', None)
+
     def _flush_host_writes(self):
         if self.__dict__.pop('_optical_dirty', False):
             opt = self.__dict__['_cache'].get('optical')

@@ -264,6 +264,13 @@ int bd_reduce(bd_ctx *ctx, int field, int op, int mask_field, double *out);
  * is_air_gap may be NULL.  out is uint8 (Nx,Ny,Nz). */
 int bd_phase_map(bd_ctx *ctx, const uint8_t *is_air_gap, uint8_t *out, int64_t n);
 
+/* Damage-dependent scattering of the scan benchmark's DOP option
+ * (validation_benchmarks.py:116-121): for every voxel not ablated,
+ * alpha = 1 - exp(-Omega), mu_s = ((1 - alpha) musp_native + alpha musp_coag) / (1 - g),
+ * in FP64 on the stored Omega.  BD_EINVAL unless the parameters are finite and
+ * g < 1; BD_ESTATE before mu_s, Omega and the masks are on the device. */
+int bd_damage_optics(bd_ctx *ctx, double musp_native, double musp_coag, double g);
+
 /* compute_phase_map on one plane (plane/index/stride as bd_download_slice),
  * with the air gap given as a per-z profile gap_z[Nz] (may be NULL); this is
  * what the per-epoch recorders need (run_highres_knowledge_suite.py:271-283)

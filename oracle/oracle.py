@@ -227,6 +227,17 @@ def plume_convolve(raw, impulse, kx, ky, kz, delta_mu, x0, y0, z0):
 _PROFILES = {'gaussian': 0, 'tophat': 1, 'kim_exp_disk': 2, 'kim_gauss_disk': 3}
 
 
+def damage_optics(mu_s, Omega, ablated, musp_native, musp_coag, g):
+    """validation_benchmarks.py:116-121 (the scan benchmark's DOP option): intact
+    voxels' mu_s from their Arrhenius damage, in float64 on the given Omega."""
+    mu_s = np.array(mu_s, dtype=np.float64, copy=True)
+    alpha = 1.0 - np.exp(-np.asarray(Omega, dtype=np.float64))
+    musp = (1.0 - alpha) * musp_native + alpha * musp_coag
+    intact = ~np.asarray(ablated, dtype=bool)
+    mu_s[intact] = musp[intact] / (1.0 - g)
+    return mu_s
+
+
 def layer_grids(Nx, Ny, Nz, Lx, Ly, Lz, layers_config, launch_axis='z'):
     """voxel_mc_3d.py:511-548: planar layer stack voxelised by cell centres."""
     dx, dy, dz = Lx / Nx, Ly / Ny, Lz / Nz

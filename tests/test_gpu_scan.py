@@ -1,0 +1,73 @@
+"""The scanned-beam liver benchmark (validation_benchmarks.py:83-141) on the device:
+the DOP step against the oracle, and one of the reference's recorded cases
+(tests/golden/shimojo_reference.json, from its shimojo_cross_sections.json)."""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.loads((pathlib.Path(__file__).parent / 'golden' / 'shimojo_reference.json').read_text())
+
+
+def test_damage_optics_matches_oracle():
+    """bd_damage_optics against the DOP loop body in float64 on the same stored
+    Omega, ablated voxels untouched: equal to the FP32 rounding of the FP64 value
+    (one ulp allowed for the device exp)."""
+    from oracle.oracle import damage_optics
+    from simulation.scan_suite import LIVER, build_engine
+    eng, _ = build_engine(10.0, 1.5, n_photons=2000, Lx=3.0, Ly=2.0, Lz=1.5)
+    rng = np.random.default_rng(5)
+    shape = (eng.Nx, eng.Ny, eng.Nz)
+    Om = rng.exponential(1.0, shape) * (rng.random(shape) < 0.6)
+    Om[0, 0, :4] = [0.0, np.log(2.0), 60.0, 1e-7]
+    eng.Omega = Om
+    abl = rng.random(shape) < 0.2
+    eng.ablated = abl
+    mu_s0 = np.asarray(eng.optical['mu_s'], dtype=np.float64)
+    eng.apply_damage_optics(LIVER['musp_native'], LIVER['musp_coag'], LIVER['g'])
+    got = np.asarray(eng.optical['mu_s'], dtype=np.float32)
+    want = damage_optics(mu_s0, np.asarray(eng.Omega, dtype=np.float32), abl, LIVER['musp_native'],
+                         LIVER['musp_coag'], LIVER['g']).astype(np.float32)
+    np.testing.assert_array_max_ulp(got, want, maxulp=1)
+    assert np.array_equal(got[abl], mu_s0[abl].astype(np.float32))
+    with pytest.raises(Exception):
+        eng.apply_damage_optics(0.5, 1.8, 1.0)
+
+
+def test_host_and_device_dop_scans_agree():
+    """The reference's host loop body (download Omega, edit optical['mu_s'], upload)
+    and the device DOP step drive the same scan to the same state, up to voxels at
+    a threshold."""
+    from simulation.scan_suite import simulate_scan
+    kw = dict(n_photons=4000, Lx=4.0, Ly=3.0, Lz=2.0, scan_margin=0.5, t_off=1.0, dt_sub=0.002, dt_epoch=0.01)
+    a = simulate_scan(50.0, 2.0, host_dop=False, **kw)
+    b = simulate_scan(50.0, 2.0, host_dop=True, **kw)
+    assert a['epochs_on'] == b['epochs_on'] == 100
+    # the two mu_s agree to an ulp and the tallies' float atomics sum in any order,
+    # so a voxel right at a threshold may flip
+    for key in ('xz_coag', 'xz_ablated'):
+        diff = int((a[key] ^ b[key]).sum())
+        assert diff <= max(2, 0.02 * int((a[key] | b[key]).sum())), (key, diff)
+    assert a['xz_coag'].sum() > 0
+
+
+@pytest.mark.parametrize('power,speed', [(10.0, 1.5), (50.0, 1.0)])
+def test_scan_case_matches_reference_record(power, speed):
+    """Two of the six recorded cases against the reference's own runs: 10 W at
+    1.5 mm/s on the 12 x 12 x 10 mm block (3 334 scan epochs, 10 s off) and 50 W at
+    1.0 mm/s on the 24 x 12 x 10 mm block (5 000 scan epochs, 30 s off), each epoch
+    20 000 photons with DOP.  All six cases reproduce the recorded metrics exactly
+    (``python -m simulation.scan_suite --all``); the bar here is one voxel (0.1 mm)
+    on each metric and 0.95 intersection over union of both cross-sections."""
+    from simulation.scan_suite import case_key, compare, run_case
+    m = run_case(power, speed)
+    d = compare(m, GOLD[case_key(power, speed)])
+    print({k: round(v, 3) for k, v in d.items()})
+    if power < 20.0:
+        assert m['vap_depth'] == 0.0
+    for k in ('vap_depth', 'vap_width', 'coag_depth', 'coag_width'):
+        assert abs(d[k]) <= 0.1 + 1e-9, (k, d)
+    assert d['iou_coag'] >= 0.95 and d['iou_ablated'] >= 0.95, d
