@@ -61,3 +61,24 @@ def test_damage_optics_closed_forms():
     a = 1.0 - math.exp(-1.0)
     assert out[3] == pytest.approx(((1 - a) * musp0 + a * muspc) / (1 - g), rel=1e-14)
     assert out[4] == 7.0  # ablated voxels keep their value
+
+
+def test_kim_depth_and_time_fit():
+    """kim_benchmark.py:37-44 and :83-118 restated: the mean lesion depth of a hand-made
+    section, and the fitted exposure on a made-up series (running maximum, linear
+    interpolation, then the first epoch at or after the fitted time)."""
+    from simulation.kim_suite import fit, mean_depth
+    m = np.zeros((5, 6), dtype=bool)
+    m[1, :2] = True
+    m[2, :4] = True
+    m[3, 3] = True  # deepest voxel counts, not the number of voxels
+    assert mean_depth(m, 0.025) == pytest.approx((2 + 4 + 4) / 3 * 0.025)
+    assert mean_depth(np.zeros_like(m), 0.025) == 0.0
+    t = [0.002 * (i + 1) for i in range(6)]
+    sim = {'t_s': t, 'coag_depth': [0.0, 0.3, 0.5, 0.45, 0.6, 0.7], 'coag_area': [0, 1, 2, 3, 4, 5],
+           'abl_depth': [0.0] * 6, 'abl_area': [0.0] * 6}
+    f = fit(sim)
+    # target 0.535 between 0.5 (t=0.006) and 0.5 (running max at t=0.008), then 0.6 at t=0.010
+    assert f['coag']['t_fit_s'] == pytest.approx(np.interp(0.535, [0.0, 0.3, 0.5, 0.5, 0.6, 0.7], t))
+    assert f['coag']['area'] == 4 and f['coag']['depth'] == 0.6
+    assert not np.isfinite(f['abl']['t_fit_s']) and f['abl']['area'] is None
