@@ -21,6 +21,14 @@
 
 namespace bd {
 
+// Arrhenius rates are evaluated as exp(ln A - Ea/(R T)): published pre-factors
+// reach 1e98 (Henriques-Moritz skin kinetics), beyond FP32, while ln A stays
+// small.  ln A is taken in FP64 on the host; A = 0 gives -inf, i.e. no damage.
+static inline float ln_prefactor(double A)
+{
+    return A > 0.0 ? (float)std::log(A) : -INFINITY;
+}
+
 struct SubArgs {
     const float *__restrict__ Tin;
     float *__restrict__ Tout;
@@ -37,7 +45,7 @@ struct SubArgs {
     int ntz, nty, ntx, xc;
     float idx2, idy2, idz2, dz, inv_h, h_eff;
     float dt, t_amb, t_abl, t_void, qscale;
-    float arrA, arrEaR, chA, chEaR, ch_tcold;
+    float arrLnA, arrEaR, chLnA, chEaR, ch_tcold;  // ln of the pre-factors (they may exceed FP32)
     int plateau, arr, chr, maskq, peak;
     double vol_dt;
     // two-sub-step kernel: H and flags are read from one buffer and written to another
@@ -61,7 +69,7 @@ struct StepK {
     int Nx, Ny, Nz;
     float idx2, idy2, idz2, dz, inv_h, h_eff;
     float dt, t_amb, t_abl, t_void, qscale;
-    float arrA, arrEaR, chA, chEaR, ch_tcold;
+    float arrLnA, arrEaR, chLnA, chEaR, ch_tcold;  // ln of the pre-factors (they may exceed FP32)
     int plateau, arr, chr, maskq;
 };
 
@@ -71,7 +79,7 @@ __device__ __forceinline__ StepK step_consts(const SubArgs &a)
     k.Nx = a.Nx; k.Ny = a.Ny; k.Nz = a.Nz;
     k.idx2 = a.idx2; k.idy2 = a.idy2; k.idz2 = a.idz2; k.dz = a.dz; k.inv_h = a.inv_h; k.h_eff = a.h_eff;
     k.dt = a.dt; k.t_amb = a.t_amb; k.t_abl = a.t_abl; k.t_void = a.t_void; k.qscale = a.qscale;
-    k.arrA = a.arrA; k.arrEaR = a.arrEaR; k.chA = a.chA; k.chEaR = a.chEaR; k.ch_tcold = a.ch_tcold;
+    k.arrLnA = a.arrLnA; k.arrEaR = a.arrEaR; k.chLnA = a.chLnA; k.chEaR = a.chEaR; k.ch_tcold = a.ch_tcold;
     k.plateau = a.plateau; k.arr = a.arr; k.chr = a.chr; k.maskq = a.maskq;
     return k;
 }
@@ -144,7 +152,7 @@ __device__ __forceinline__ bool char_active(const StepK &a, float Tn, uint8_t fl
 __device__ __forceinline__ float char_step(const StepK &a, float Tn, float chi)
 {
 #pragma clang fp contract(off)
-    const float rate = a.chA * expf(-a.chEaR / (Tn + 273.15f));
+    const float rate = expf(a.chLnA - a.chEaR / (Tn + 273.15f));
     return 1.0f - (1.0f - chi) * expf(-rate * a.dt);
 }
 
@@ -152,7 +160,7 @@ __device__ __forceinline__ float char_step(const StepK &a, float Tn, float chi)
 __device__ __forceinline__ float arr_step(const StepK &a, float Tn, float om)
 {
 #pragma clang fp contract(off)
-    return om + a.dt * a.arrA * expf(-a.arrEaR / (Tn + 273.15f));
+    return om + a.dt * expf(a.arrLnA - a.arrEaR / (Tn + 273.15f));
 }
 
 // Arrhenius damage and char kinetics of one cell at its new temperature
@@ -545,9 +553,9 @@ static void fill_args(Ctx *c, SubArgs &a, int is_active, int flags)
     a.t_abl = (float)p.t_abl;
     a.t_void = (float)p.t_void;
     a.qscale = is_active ? 1.0f : 0.0f;
-    a.arrA = (float)p.arr_A;
+    a.arrLnA = ln_prefactor(p.arr_A);
     a.arrEaR = (float)p.arr_Ea_over_R;
-    a.chA = (float)p.char_A;
+    a.chLnA = ln_prefactor(p.char_A);
     a.chEaR = (float)p.char_Ea_over_R;
     // temperature below which one step's rate dt stays under 2^-26:
     // A dt e^{-Ea/(R Tk)} < 2^-26  <=>  Tk < (Ea/R) / ln(A dt 2^26), with 0.1% margin
@@ -769,19 +777,19 @@ __global__ void k_plateau(float *__restrict__ T, float *__restrict__ H, const fl
 }
 
 __global__ void k_arrhenius(const float *__restrict__ T, float *__restrict__ Om, const uint8_t *__restrict__ flags,
-                            float A, float EaR, float dt, int64_t n)
+                            float lnA, float EaR, float dt, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        if (!(flags[i] & F_ABL)) Om[i] += dt * A * expf(-EaR / (T[i] + 273.15f));
+        if (!(flags[i] & F_ABL)) Om[i] += dt * expf(lnA - EaR / (T[i] + 273.15f));
 }
 
 __global__ void k_char(const float *__restrict__ T, float *__restrict__ chi, const uint8_t *__restrict__ flags,
-                       float A, float EaR, float dt, int64_t n)
+                       float lnA, float EaR, float dt, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint8_t fl = flags[i];
         if ((fl & F_DRY) && !(fl & F_ABL)) {
-            const float rate = A * expf(-EaR / (T[i] + 273.15f));
+            const float rate = expf(lnA - EaR / (T[i] + 273.15f));
             chi[i] = 1.0f - (1.0f - chi[i]) * expf(-rate * dt);
         }
     }
@@ -800,15 +808,15 @@ int launch_plateau(Ctx *c)
 int launch_arrhenius(Ctx *c, double A, double EaR, double dt)
 {
     hipLaunchKernelGGL(k_arrhenius, dim3(ew_blocks(c->nvox)), dim3(256), 0, c->stream, c->T, c->Om, c->flags,
-                       (float)A, (float)EaR, (float)dt, c->nvox);
+                       ln_prefactor(A), (float)EaR, (float)dt, c->nvox);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
 
 int launch_char(Ctx *c, double A, double EaR, double dt)
 {
-    hipLaunchKernelGGL(k_char, dim3(ew_blocks(c->nvox)), dim3(256), 0, c->stream, c->T, c->chi, c->flags, (float)A,
-                       (float)EaR, (float)dt, c->nvox);
+    hipLaunchKernelGGL(k_char, dim3(ew_blocks(c->nvox)), dim3(256), 0, c->stream, c->T, c->chi, c->flags,
+                       ln_prefactor(A), (float)EaR, (float)dt, c->nvox);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
