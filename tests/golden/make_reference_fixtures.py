@@ -23,6 +23,7 @@ KEEP = {
     "C3-R-0", "C3-R-15", "C3-R-30", "C3-R-45", "C3-R-60",
     "C3-z-0", "C3-z-15", "C3-z-30", "C3-z-45", "C3-z-60",
     "V3a", "V3b", "V4-2ms", "V4-10ms",
+    "V2c-a0.5", "V2c-a0.7", "V2c-a0.9", "V2c-a0.95", "V2c-a0.99",
     # coupled engine: crater depth of the McMillan et al. pixel runs (mcmillan_benchmark.py:60-99)
     "M-50-exp", "M-100-exp", "M-150-exp", "M-200-exp", "M-250-exp", "M-300-exp", "M-350-exp", "M-400-exp",
 }
