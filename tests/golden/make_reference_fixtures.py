@@ -24,6 +24,7 @@ KEEP = {
     "C3-z-0", "C3-z-15", "C3-z-30", "C3-z-45", "C3-z-60",
     "V3a", "V3b", "V4-2ms", "V4-10ms",
     "V2c-a0.5", "V2c-a0.7", "V2c-a0.9", "V2c-a0.95", "V2c-a0.99",
+    "V2d-a0.5", "V2d-a0.7", "V2d-a0.9", "V2d-a0.99",
     # coupled engine: crater depth of the McMillan et al. pixel runs (mcmillan_benchmark.py:60-99)
     "M-50-exp", "M-100-exp", "M-150-exp", "M-200-exp", "M-250-exp", "M-300-exp", "M-350-exp", "M-400-exp",
 }
@@ -40,6 +41,8 @@ def main(ref_root):
             s = r.get("series") or {}
             if r["id"] in ("C2-dark-end", "C2-shiny-end"):
                 rec["series"] = {"t": s["t"], "dT": s["dT"]}
+            if r["id"].startswith("V2d-"):
+                rec["series"] = {"se_ref": s["se_ref"]}
             if r["id"] in ("V4-2ms", "V4-10ms"):
                 rec["series"] = {"t_s": s["t_s"], "depth_mm": s["depth_mm"]}
             out[r["id"]] = rec

@@ -209,3 +209,26 @@ def test_semi_infinite_albedo_sweep_gpu(albedo):
                                            'g': 0.0, 'n': 1.0}])['spectro_raw']['R_diff'] for s in range(5)]
     m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(5)
     assert abs(m - gold["reference_value"]) < 4 * se + 2e-4, (m, se, gold["reference_value"])
+
+
+@pytest.mark.parametrize("albedo", [0.5, 0.7, 0.9, 0.99])
+def test_semi_infinite_mismatched_albedo_sweep_gpu(albedo):
+    """V2d (verification_benchmarks.py:406-424): total reflectance (specular + diffuse)
+    of a semi-infinite isotropic scatterer with n = 1.5 against the reference's
+    independent 1-D Monte Carlo (8 x 400 000 photons, its mean and standard error
+    recorded), over 5 x 100 000 photons, within 4 combined standard errors + 2e-4."""
+    import json
+    import os
+    from simulation.voxel_mc_3d import run_voxel_mc3d
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "reference_engine_values.json")))["values"][f"V2d-a{albedo}"]
+    mu_t = 10.0
+    vals = []
+    for s in range(5):
+        r = run_voxel_mc3d(n_photons=100000, spot_dia=0.01, Lx=60.0, Ly=60.0, Lz=30.0, Nx=6, Ny=6, Nz=30,
+                           seed=31 + s, layers_config=[{'z_end': 30.0, 'mu_a': mu_t * (1 - albedo),
+                                                        'mu_s': mu_t * albedo, 'g': 0.0, 'n': 1.5}])
+        vals.append(r['spectro_raw']['R_diff'] + r['spectro_raw']['R_spec'])
+    m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(5)
+    bar = 4 * np.hypot(se, gold["series"]["se_ref"]) + 2e-4
+    assert abs(m - gold["reference_value"]) < bar, (m, se, gold["reference_value"])
