@@ -709,8 +709,11 @@ class OracleEngine:
                 self._inject(newly)
         return n_new
 
-    def step_epoch(self):
-        """coupled_ablation.py:440-617."""
+    # -- one epoch, in the reference's order (coupled_ablation.py:440-617), split in three
+    #    parts so a test can drive the second and third from given fields:
+    #    optics (MC or a given Q), sub-steps, then phase update / removal / plume.
+    def _optics(self):
+        """coupled_ablation.py:442-486: MC of the current optics (+ plume), Q, Q_thermal."""
         t0 = time.perf_counter()
         if self.power > 0.0:
             if self.plume is not None and np.any(self.plume.mu_ext_plume > 0.0):
@@ -741,8 +744,24 @@ class OracleEngine:
             self.Phi = np.zeros_like(self.T)
             Q_th = self.Q
             absorbed, closure = 0.0, 1.0
-        t_opt = time.perf_counter() - t0
-        t1 = time.perf_counter()
+        return Q_th, absorbed, closure, time.perf_counter() - t0
+
+    def _boundary_loss(self, T):
+        """Energy [J] leaving through the top and bottom faces in one sub-step: the
+        boundary term of voxel_thermal_3d.py:97,105 (half-cell conduction in series
+        with h_eff), summed over the z = 0 and z = Nz-1 planes (both for Nz = 1)."""
+        if self.h_eff <= 0.0:
+            return 0.0
+        loss = 0.0
+        for kk in (0, self.Nz - 1):
+            kc = self.k_grid[:, :, kk]
+            rser = self.dz * (0.5 * self.dz / np.maximum(kc, 1e-30) + 1.0 / max(self.h_eff, 1e-30))
+            loss += float(np.sum((T[:, :, kk] - self.t_ambient) / rser))
+        return loss * self.voxel_vol * self.dt_sub
+
+    def run_substeps(self, Q_th):
+        """coupled_ablation.py:487-545: n_substeps fused steps (stencil, plateau, Arrhenius,
+        char, void-confined plume diffusion), then T_peak.  Returns the conducted energy [J]."""
         has_diff = self.plume is not None and self.plume.model == 'diffusion'
         void = self.compute_void_mask() if has_diff else self.ablated
         ab8 = self.ablated.view(np.uint8)
@@ -760,7 +779,9 @@ class OracleEngine:
                 if d_rate > 1e-6:
                     dt_p = min(dt_p, 0.45 / (2.0 * d_rate))
         N = self.Nx, self.Ny, self.Nz
+        conducted = 0.0
         for _ in range(self.n_substeps):
+            conducted += self._boundary_loss(T)
             thermal_step(T, Tn, Q_th, self.k_grid, self.rho_cp_grid, *N, self.dx, self.dy, self.dz, self.dt_sub,
                          self.h_eff, self.t_ambient, True)
             plateau(Tn, self.H_acc, self.rho_cp_grid, ab8, self.t_abl, *N, dr8, self.t_void)
@@ -778,10 +799,17 @@ class OracleEngine:
             if P is not self.plume.raw_field:
                 self.plume.raw_field[:] = P
         np.maximum(self.T_peak, self.T, out=self.T_peak)
-        t_th = time.perf_counter() - t1
+        return conducted
+
+    def finish_epoch(self):
+        """coupled_ablation.py:547-617: time step, phase update (or the single-stage
+        test), _remove with air-gap perforation, plume sync and centre-column diagnostics.
+        Returns (n_new, ablated energy [J], plume tau, transmission, peak, mean)."""
         self.t += self.dt_epoch
         self.epoch += 1
         rem = self._to_remove
+        ab8 = self.ablated.view(np.uint8)
+        dr8 = self.dry.view(np.uint8)
         if self.dry_state is None:
             single_stage(self.H_acc, self.h_lat, self.T, self.t_abl, ab8, rem.view(np.uint8))
         else:
@@ -791,6 +819,9 @@ class OracleEngine:
                          self.optical['mu_s'], self.optical['g'], self.optical['n'], self.k_grid,
                          self.rho_cp_grid, self._wet[0], self._wet[1], self._wet[2], self._ds, self._cs,
                          self._cs_chi_remove, self.alpha_max_stable, rem.view(np.uint8))
+        # energy the removed cells held above ambient, with their properties after the update
+        ablated_J = float(np.sum(self.rho_cp_grid[rem] * (self.T[rem] - self.t_ambient) + self.H_acc[rem])) \
+            * self.voxel_vol
         n_new = self._remove(rem)
         if self.plume is not None:
             vm = self._to_plume(self.compute_void_mask())
@@ -806,9 +837,65 @@ class OracleEngine:
             mean = float(np.mean(self._from_plume(mua_p)[self.ablated])) if np.any(self.ablated) else 0.0
         else:
             tau, trans, peak, mean = 0.0, 1.0, 0.0, 0.0
+        return n_new, ablated_J, tau, trans, peak, mean
+
+    def _epoch(self, Q_th, absorbed, closure, t_opt):
+        absorbed_J = float(np.sum(Q_th)) * self.voxel_vol * self.dt_epoch
+        t1 = time.perf_counter()
+        conducted_J = self.run_substeps(Q_th)
+        t_th = time.perf_counter() - t1
+        n_new, ablated_J, tau, trans, peak, mean = self.finish_epoch()
+        self.ledger_J = {'absorbed': absorbed_J, 'conducted': conducted_J, 'ablated': ablated_J}
         return {'epoch': self.epoch, 'time_s': self.t, 'n_new': n_new, 'absorbed_frac': absorbed,
                 'mc_energy_closure': closure, 't_optical_s': t_opt, 't_thermal_s': t_th,
-                'plume_transmission': trans, 'plume_tau': tau, 'plume_peak_mu_a': peak, 'plume_mean_mu_a': mean}
+                'plume_transmission': trans, 'plume_tau': tau, 'plume_peak_mu_a': peak, 'plume_mean_mu_a': mean,
+                'ledger_J': dict(self.ledger_J)}
+
+    def step_epoch(self):
+        """coupled_ablation.py:440-617."""
+        return self._epoch(*self._optics())
+
+    def step_epoch_with_q(self, Q):
+        """The epoch from the sub-steps on (coupled_ablation.py:476-617) with a given
+        absorbed-power field Q [W/mm^3] in place of the transport: Q is zeroed in the
+        void for the sub-steps, as after the MC.  ``absorbed_frac`` is the absorbed
+        power over the beam power and the closure is NaN."""
+        self.Q = np.array(Q, dtype=np.float64, copy=True)
+        self.Q_opt = self.Q
+        self.Phi = np.zeros_like(self.T)
+        Q_th = self.Q.copy()
+        if np.any(self.ablated):
+            Q_th[self.ablated] = 0.0
+        absorbed = float(np.sum(Q_th)) * self.voxel_vol / self.power if self.power > 0.0 else 0.0
+        return self._epoch(Q_th, absorbed, float('nan'), 0.0)
+
+    def state_energy_J(self):
+        """Heat held above ambient by the intact cells: sum of V (rho_cp (T - T_amb) + H_acc)."""
+        live = ~self.ablated
+        return float(np.sum(self.rho_cp_grid[live] * (self.T[live] - self.t_ambient) + self.H_acc[live])) \
+            * self.voxel_vol
+
+    def load_state(self, **fields):
+        """Overwrite state arrays in place (e.g. with a device engine's FP32 state as FP64),
+        so one epoch of this engine can be compared from an identical start.  Keys: T,
+        H_acc, chi, Omega, T_peak, ablation_time, k_grid, rho_cp_grid, h_lat, dry, ablated,
+        mu_a, mu_s, g, n, wet (3-tuple), plume_raw, plume_ext."""
+        for name, v in fields.items():
+            if name in ('dry', 'ablated'):
+                getattr(self, name)[...] = np.asarray(v, dtype=np.bool_)
+            elif name in ('mu_a', 'mu_s', 'g', 'n'):
+                self.optical[name][...] = np.asarray(v, dtype=np.float64)
+            elif name == 'wet':
+                self._wet = tuple(np.array(a, dtype=np.float64, copy=True) for a in v)
+            elif name == 'plume_raw':
+                self.plume.raw_field[...] = np.asarray(v, dtype=np.float32)
+            elif name == 'plume_ext':
+                self.plume.mu_ext_plume[...] = np.asarray(v, dtype=np.float32)
+            elif name in ('T', 'H_acc', 'chi', 'Omega', 'T_peak', 'ablation_time', 'k_grid', 'rho_cp_grid',
+                          'h_lat'):
+                getattr(self, name)[...] = np.asarray(v, dtype=np.float64)
+            else:
+                raise KeyError(name)
 
     def centerline_depth_mm(self, ix=None, iy=None, k_surface=0):
         """coupled_ablation.py:662-667."""

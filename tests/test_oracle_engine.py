@@ -1,0 +1,258 @@
+"""Pins of the oracle's coupled-engine glue (OracleEngine, compute_phase_map) on the CPU.
+
+The engine glue restates coupled_ablation.py:440-667 and run_multispectral_matrix.py:168-192.
+It is pinned here by
+  * the reference engine's own recorded coupled runs: the staged-ablation series
+    (sagi_stages.json) and the McMillan pixel crater depths (benchmark_results.json, M-<E>-exp);
+  * truth tables, written out case by case, for compute_phase_map, compute_void_mask and
+    _remove with its air-gap perforation;
+  * energy conservation of the epoch with the FP64 ledger (absorbed - conducted - ablated equals
+    the change of the heat held by the intact cells), which fixes the ledger's definitions.
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+# McMillan et al. tissue constants (mcmillan_benchmark.py:32-41, sagi_stages.py:26-55)
+W0, PRO0 = 0.75, 0.25
+RHO_WET = 1000.0 / (W0 + 0.649 * PRO0)
+CP_WET = 1000.0 * (4.2 * W0 + 1.09 * PRO0)
+K_WET = RHO_WET * (6.28e-4 * W0 + 1.17e-4 * PRO0)
+RHO_DRY, CP_DRY = 1000.0 / 0.649, 1000.0 * 1.09
+K_DRY = RHO_DRY * 1.17e-4
+MU_A_WET, H_LAT = 85.0, W0 * RHO_WET * 2.26e6 * 1e-9
+
+
+def _absorber_kwargs(shp, L, Lz, t_remove, **kw):
+    out = dict(optical_props={'mu_a': np.full(shp, MU_A_WET), 'mu_s': np.zeros(shp), 'g': np.zeros(shp),
+                              'n': np.ones(shp)},
+               k_grid=np.full(shp, K_WET * 1e-3), rho_cp_grid=np.full(shp, RHO_WET * CP_WET * 1e-9),
+               h_lat_grid=np.full(shp, H_LAT), Lx=L, Ly=L, Lz=Lz, t_abl=100.0, h_eff=0.0,
+               dry_state=dict(mu_a=0.25 * MU_A_WET, mu_s=0.0, g=0.0, n=1.0, k=K_DRY * 1e-3,
+                              rho_cp=RHO_DRY * CP_DRY * 1e-9, t_remove=t_remove))
+    out.update(kw)
+    return out
+
+
+def test_oracle_engine_staged_ablation_series_matches_reference_record(oracle):
+    """sagi_stages.py:26-55 on the oracle engine against the reference engine's recorded series:
+    the same stage-time, surface-temperature and depth bars as the device test."""
+    ref = json.load(open(os.path.join(GOLDEN, "sagi_stages_reference.json")))
+    dz, Nz, dt_epoch, Lx = 0.005, 100, 0.002, 0.2
+    w0 = 10.0 * (Lx / 2.0)
+    shp = (4, 4, Nz)
+    eng = oracle.OracleEngine(**_absorber_kwargs(shp, Lx, Nz * dz, 300.0, power=0.5 * np.pi * w0 ** 2 / 2.0,
+                                                 spot_dia=2 * w0, t_ambient=25.0, dt_epoch=dt_epoch, dt_sub=4e-5,
+                                                 n_photons=60000, seed=3, mc_threads=4))
+    rec = {k: [] for k in ('t_s', 'T_surface', 'depth_mm', 'n_dry_layers')}
+    for _ in range(int(round(1.2 / dt_epoch))):
+        eng.step_epoch()
+        col_abl = eng.ablated.all(axis=(0, 1))
+        k_surf = int(np.argmin(col_abl)) if not col_abl.all() else Nz - 1
+        rec['t_s'].append(eng.t)
+        rec['T_surface'].append(float(eng.T[:, :, k_surf].mean()))
+        rec['depth_mm'].append(float(col_abl.sum() * dz))
+        rec['n_dry_layers'].append(int(eng.dry.any(axis=(0, 1)).sum()))
+    r = {k: np.asarray(v) for k, v in rec.items()}
+    t = np.asarray(ref['t_s'])
+    np.testing.assert_allclose(r['t_s'], t, atol=1e-9)
+    Tr, dr, ndr = (np.asarray(ref[k]) for k in ('T_surface', 'depth_mm', 'n_dry_layers'))
+
+    def first(mask):
+        return float(t[np.argmax(mask)]) if mask.any() else float('inf')
+    for name, a, b in (("boiling", r['T_surface'] >= 99.9, Tr >= 99.9),
+                       ("first dry layer", r['n_dry_layers'] > 0, ndr > 0),
+                       ("first removal", r['depth_mm'] > 0, dr > 0)):
+        print(name, first(a), first(b))
+        assert abs(first(a) - first(b)) <= 0.1 * first(b) + 0.004, name
+    dT = np.abs(r['T_surface'] - Tr)
+    print("surface T: median |dT|", float(np.median(dT)), "mean", float(dT.mean()))
+    assert float(np.median(dT)) < 1.0 and float(dT.mean()) < 3.0
+    dd = np.abs(r['depth_mm'] - dr)
+    assert float(dd.max()) <= 0.0101 and abs(r['depth_mm'][-1] - dr[-1]) <= 0.0051
+
+
+@pytest.mark.parametrize("E", [50, 400])
+def test_oracle_engine_mcmillan_depths_match_reference_engine(oracle, E):
+    """mcmillan_benchmark.py:46-99 on the oracle engine: the crater depth of the triangular
+    E-mJ pulse against the reference engine's recorded model value (M-<E>-exp), within one voxel
+    (the photon streams differ from the reference's unseeded ones)."""
+    ref = json.load(open(os.path.join(GOLDEN, "reference_engine_values.json")))["values"]
+    N, voxel, dt = 81, 0.0075, 0.002
+    Nz = int(round(1.2 / voxel))
+    n_ep = int(round(1.2e-3 * E / dt))
+    t = (np.arange(n_ep) + 0.5) * dt
+    p = np.clip(1.0 - np.abs(2.0 * t / (1.2e-3 * E) - 1.0), 0.0, None)
+    P = p * (E * 1e-3) / (p.sum() * dt)
+    eng = oracle.OracleEngine(**_absorber_kwargs((N, N, Nz), N * voxel, Nz * voxel, 500.0, power=1.0,
+                                                 spot_dia=0.25, t_ambient=5.0, dt_epoch=dt, dt_sub=4e-5,
+                                                 n_photons=20000, beam_profile='tophat', seed=E, mc_threads=8))
+    for i in range(n_ep):
+        eng.power = float(P[i])
+        eng.step_epoch()
+    d = eng.centerline_depth_mm(N // 2, N // 2)
+    print(E, d, ref[f"M-{E}-exp"]["model"])
+    assert abs(d - ref[f"M-{E}-exp"]["model"]) <= voxel + 1e-9
+
+
+# ------------------------------------------------------------------ phase map truth table
+def _phase_table():
+    """Every combination of (T >= 99.5, H > 0, dry, chi >= 0.1, ablated, air gap) with the phase
+    the reference's map assigns (run_multispectral_matrix.py:168-192): void wins over everything,
+    then the air gap (void, and neither boiling nor char), then char, then boiling, else intact."""
+    rows = []
+    for hot, banked, dry, charred, abl, gap in itertools.product((0, 1), repeat=6):
+        if abl or gap:
+            expect = 3
+        elif dry and charred:
+            expect = 1
+        elif hot or banked:
+            expect = 2
+        else:
+            expect = 0
+        rows.append((hot, banked, dry, charred, abl, gap, expect))
+    return rows
+
+
+def test_oracle_phase_map_truth_table(oracle):
+    rows = _phase_table()
+    shape = (4, 4, 4)
+    T = np.array([99.5 if r[0] else np.nextafter(99.5, 0.0) for r in rows]).reshape(shape)
+    H = np.array([1e-30 if r[1] else 0.0 for r in rows]).reshape(shape)
+    dry = np.array([bool(r[2]) for r in rows]).reshape(shape)
+    chi = np.array([0.10 if r[3] else np.nextafter(0.10, 0.0) for r in rows]).reshape(shape)
+    abl = np.array([bool(r[4]) for r in rows]).reshape(shape)
+    gap = np.array([bool(r[5]) for r in rows]).reshape(shape)
+    expect = np.array([r[6] for r in rows], dtype=np.uint8).reshape(shape)
+    np.testing.assert_array_equal(oracle.compute_phase_map(T, chi, dry, abl, H, gap), expect)
+    # without an air gap mask, the gap column of the table does not apply
+    expect_nogap = np.array([3 if r[4] else (1 if r[2] and r[3] else (2 if r[0] or r[1] else 0))
+                             for r in rows], dtype=np.uint8).reshape(shape)
+    np.testing.assert_array_equal(oracle.compute_phase_map(T, chi, dry, abl, H, None), expect_nogap)
+
+
+# ------------------------------------------------------------------ void mask and removal
+def _tiny_engine(oracle, air_gap_range, shape=(3, 2, 8)):
+    ones = np.ones(shape)
+    return oracle.OracleEngine(
+        optical_props={'mu_a': 2.0 * ones, 'mu_s': 5.0 * ones, 'g': 0.8 * ones, 'n': 1.4 * ones},
+        k_grid=0.3e-3 * ones, rho_cp_grid=3.0e-3 * ones, h_lat_grid=2.0 * ones, Lx=0.03, Ly=0.02, Lz=0.08,
+        power=0.0, spot_dia=0.01, t_abl=100.0, t_ambient=20.0, dt_epoch=1e-3, dt_sub=5e-5, n_photons=1,
+        air_gap_range=air_gap_range, dry_state={'t_remove': 350.0, 'rho_cp': 2.5e-3}, seed=0)
+
+
+def test_oracle_void_mask_truth_table(oracle):
+    """compute_void_mask (coupled_ablation.py:619-626): the void is the ablated set, and the whole
+    air gap [k0, k1) joins it, over every column, once any cell of plane k0-1 is ablated."""
+    e = _tiny_engine(oracle, (3, 6))
+    np.testing.assert_array_equal(e.compute_void_mask(), np.zeros(e.ablated.shape, bool))
+    e.ablated[0, 0, 1] = True            # above the cover plane: no breach
+    e.ablated[2, 1, 4] = True            # inside the gap only: no breach
+    np.testing.assert_array_equal(e.compute_void_mask(), e.ablated)
+    e.ablated[1, 0, 2] = True            # plane k0-1 = 2 reached: breach
+    v = e.compute_void_mask()
+    expect = e.ablated.copy()
+    expect[:, :, 3:6] = True
+    np.testing.assert_array_equal(v, expect)
+    # a gap that starts at the top plane never breaches; no gap means the ablated set
+    for rng_ in ((0, 3), None):
+        e2 = _tiny_engine(oracle, rng_)
+        e2.ablated[:, :, 0] = True
+        np.testing.assert_array_equal(e2.compute_void_mask(), e2.ablated)
+
+
+def test_oracle_remove_and_perforation_truth_table(oracle):
+    """_remove (coupled_ablation.py:628-660): a removed cell becomes void air (optics 0/0/0/1,
+    k_void, rho_cp_void, T = t_void, H = 0, chi = 0, not dry, ablation time t); when removal
+    happens, every column ablated at plane k0-1 whose plane k0 is not yet ablated is opened
+    through [k0, k1) with the same time, and nothing else of those gap cells changes."""
+    e = _tiny_engine(oracle, (3, 6))
+    e.t = 0.004
+    e.ablated[1, 1, 2] = True             # a column opened at k0-1 earlier but never perforated
+    e.ablated[2, 0, 2:6] = True           # a column already perforated
+    e.ablation_time[2, 0, 2:6] = 0.001
+    e.dry[0, 0, 2] = True
+    e.chi[0, 0, 2] = 0.7
+    e.H_acc[0, 0, 2] = 0.3
+    e.T[...] = 42.0
+    before = {k: np.array(getattr(e, k), copy=True) for k in ('T', 'H_acc', 'chi', 'dry', 'k_grid', 'rho_cp_grid')}
+    opt_before = {k: v.copy() for k, v in e.optical.items()}
+    abl_before = e.ablated.copy()
+
+    # no removal: nothing happens, not even the pending perforation
+    assert e._remove(np.zeros(e.ablated.shape, bool)) == 0
+    np.testing.assert_array_equal(e.ablated, abl_before)
+
+    newly = np.zeros(e.ablated.shape, bool)
+    newly[0, 0, 2] = True
+    assert e._remove(newly) == 1
+    expect_abl = abl_before.copy()
+    expect_abl[0, 0, 2] = True
+    expect_abl[0, 0, 3:6] = True          # the new opening is perforated
+    expect_abl[1, 1, 3:6] = True          # the pending one too
+    np.testing.assert_array_equal(e.ablated, expect_abl)
+    expect_time = np.full(e.ablated.shape, -1.0)
+    expect_time[2, 0, 2:6] = 0.001        # untouched: plane k0 was already open
+    expect_time[0, 0, 2:6] = 0.004
+    expect_time[1, 1, 3:6] = 0.004        # the cover cell itself kept its (initial) time
+    np.testing.assert_array_equal(e.ablation_time, expect_time)
+    # the removed cell
+    assert (e.optical['mu_a'][0, 0, 2], e.optical['mu_s'][0, 0, 2], e.optical['g'][0, 0, 2],
+            e.optical['n'][0, 0, 2]) == (0.0, 0.0, 0.0, 1.0)
+    assert e.k_grid[0, 0, 2] == e.k_void and e.rho_cp_grid[0, 0, 2] == e.rho_cp_void
+    assert e.T[0, 0, 2] == e.t_void == 350.0 and e.H_acc[0, 0, 2] == 0.0 and e.chi[0, 0, 2] == 0.0
+    assert not e.dry[0, 0, 2]
+    # every other cell, the perforated gap cells included, keeps its state
+    other = np.ones(e.ablated.shape, bool)
+    other[0, 0, 2] = False
+    for k, v in before.items():
+        np.testing.assert_array_equal(getattr(e, k)[other], v[other], err_msg=k)
+    for k, v in opt_before.items():
+        np.testing.assert_array_equal(e.optical[k][other], v[other], err_msg=k)
+
+
+# ------------------------------------------------------------------ energy ledger
+def _ledger_slab(oracle_or_engine_cls, shape=(12, 10, 16), **over):
+    """A wet slab in the single-stage model (removal when H >= h_lat and T >= t_abl) with a
+    surface loss and thermally inert void (k_void = 0): the FP64 ledger then closes exactly."""
+    kw = dict(optical_props={'mu_a': np.full(shape, 2.0), 'mu_s': np.zeros(shape), 'g': np.zeros(shape),
+                             'n': np.ones(shape)},
+              k_grid=np.full(shape, 0.5e-3), rho_cp_grid=np.full(shape, 3.8e-3), h_lat_grid=np.full(shape, 0.6),
+              Lx=shape[0] * 0.02, Ly=shape[1] * 0.02, Lz=shape[2] * 0.015, power=6.0, spot_dia=0.1, t_abl=100.0,
+              t_ambient=20.0, dt_epoch=1e-3, dt_sub=5e-5, n_photons=1, h_eff=2e-3, k_void=0.0, seed=0)
+    kw.update(over)
+    return kw
+
+
+def ledger_q(shape, dz, scale=150.0):
+    """A fixed absorbed-power field [W/mm^3]: a Gaussian column decaying with depth."""
+    Nx, Ny, Nz = shape
+    x = (np.arange(Nx) - (Nx - 1) / 2.0)[:, None, None]
+    y = (np.arange(Ny) - (Ny - 1) / 2.0)[None, :, None]
+    z = np.arange(Nz)[None, None, :] * dz
+    return scale * np.exp(-(x ** 2 + y ** 2) / 8.0) * np.exp(-z / 0.05)
+
+
+def test_oracle_ledger_closes_energy_balance(oracle):
+    """absorbed - conducted - ablated = change of sum V (rho_cp (T - T_amb) + H) over the intact
+    cells, epoch by epoch, through boiling and removal (FP64, to round-off)."""
+    kw = _ledger_slab(oracle)
+    e = oracle.OracleEngine(**kw)
+    Q = ledger_q(e.T.shape, e.dz)
+    E0 = e.state_energy_J()
+    removed = 0
+    for ep in range(14):
+        d = e.step_epoch_with_q(Q)
+        L = d['ledger_J']
+        E1 = e.state_energy_J()
+        bal = L['absorbed'] - L['conducted'] - L['ablated']
+        assert abs((E1 - E0) - bal) <= 1e-9 * L['absorbed'], (ep, E1 - E0, bal)
+        assert L['conducted'] > 0.0
+        removed += d['n_new']
+        E0 = E1
+    assert removed > 0 and np.any(e.H_acc > 0.0)
