@@ -46,7 +46,7 @@ import time
 import numpy as np
 
 from .coupled_ablation import CoupledAblationEngine
-from .run_multispectral_matrix import CHAR_STATE, H_LAT_WATER
+from .phase_map import CHAR_STATE, H_LAT_WATER
 from .run_record import RunRecorder, analyze_npz
 from .tissue_stack import BASE_LAYERS, eval_single_layer
 

@@ -1,9 +1,10 @@
-"""Constants and the phase map of ``simulation/run_multispectral_matrix.py``.
+"""The four-phase tissue map and the char / latent-heat constants of the beetle runners.
 
-Only the parts of the reference's matrix runner that the engine callers import
-are kept here (:54, :69-78, :168-192): ``H_LAT_WATER``, ``CHAR_STATE`` and
-``compute_phase_map``.  The beetle model builder needs the tissue-optics
-package, which is outside this build.
+``compute_phase_map`` is ``run_multispectral_matrix.py:168-192`` evaluated on the device
+(``bd_phase_map``); ``H_LAT_WATER`` and ``CHAR_STATE`` are the constants the runners pass to the
+engine (run_multispectral_matrix.py:54, 69-78).  The matrix runner itself is a caller of the
+engine, not part of it: with the reference checkout on the package path (see
+``simulation/__init__.py``) ``simulation.run_multispectral_matrix`` loads from there unchanged.
 """
 
 import numpy as np

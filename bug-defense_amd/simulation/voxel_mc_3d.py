@@ -136,12 +136,16 @@ def run_voxel_mc3d(
     engine: str = 'auto',
     seed: int = None,
     score_fluence: bool = True,
+    n_ambient: float = 1.0,
+    max_events: int = _MAX_EVENTS_PER_PHOTON,
 ):
     """Voxel MC of voxel_mc_3d.py:425-717 on the gfx950 transport kernel.
 
     Extra keywords (not in the reference): ``seed`` keys the Philox streams
     (random when None, like the reference's unseeded generators) and
-    ``score_fluence=False`` skips the path-length tally (``Phi_3d`` is then zero).
+    ``score_fluence=False`` skips the path-length tally (``Phi_3d`` is then zero);
+    ``n_ambient`` is the index outside the domain and ``max_events`` the per-photon event
+    budget (voxel_mc_3d.py:30; the Metal entry point passes its ``max_steps``).
     """
     check_engine(engine)
     dx, dy, dz = Lx / Nx, Ly / Ny, Lz / Nz
@@ -160,7 +164,8 @@ def run_voxel_mc3d(
         ctx.upload(name, a)
     centre = beam_centre(Lx, Ly, Lz, la, beam_center)
     beam = make_beam(power, n_photons, spot_dia, theta_inc_deg, phi_inc_deg, centre, r_max, la, beam_profile, seed,
-                     score_fluence=score_fluence)
+                     score_fluence=score_fluence, max_events=max_events)
+    beam.n_ambient = float(n_ambient)
     t0 = time.perf_counter()
     res = ctx.mc_run(beam)
     elapsed = time.perf_counter() - t0

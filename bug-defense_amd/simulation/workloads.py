@@ -13,7 +13,7 @@ from __future__ import annotations
 import numpy as np
 
 from .coupled_ablation import K_AIR, RHO_CP_AIR
-from .run_multispectral_matrix import CHAR_STATE, H_LAT_WATER
+from .phase_map import CHAR_STATE, H_LAT_WATER
 
 # thicknesses [um] of the 7 planar layers; index 1 is the air gap
 STACK_UM = (20.0, 300.0, 80.0, 150.0, 200.0, 400.0, 60.0)

@@ -59,7 +59,7 @@ def test_phase_update_masks_bit_exact(oracle):
 
 def test_single_stage_and_phase_map_bit_exact(oracle):
     from simulation.coupled_ablation import _update_single_stage_ablation_numba
-    from simulation.run_multispectral_matrix import compute_phase_map
+    from simulation.phase_map import compute_phase_map
     shape = (17, 13, 29)
     H, hl, T, chi, dry, abl, opt, k, rc = _phase_inputs(4, shape)
     rem_o = np.zeros(shape, np.uint8)
@@ -184,7 +184,7 @@ def test_engine_against_oracle_engine(oracle, dt_clamp):
     the FP64 oracle engine trace the same photon streams; compare the state after 6 epochs.
     With dt_clamp the plume takes the Metal solver's diffusion step on both sides."""
     from simulation.coupled_ablation import CoupledAblationEngine
-    from simulation.run_multispectral_matrix import CHAR_STATE
+    from simulation.phase_map import CHAR_STATE
     shape = (32, 32, 40)
     rng = np.random.default_rng(0)
     mu_a = np.zeros(shape)

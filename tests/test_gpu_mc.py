@@ -118,7 +118,7 @@ def test_semi_infinite_h_function_gpu():
     assert abs(m - 0.414947479) < 4 * se + 2e-4
 
 
-@pytest.mark.parametrize("profile", ["gaussian", "tophat", "kim_exp_disk"])
+@pytest.mark.parametrize("profile", ["gaussian", "tophat", "kim_exp_disk", "kim_gauss_disk"])
 def test_beam_profiles_gpu_vs_oracle(oracle, profile):
     """Launch footprint (voxel_mc_3d.py:101-161): surface-absorber moments vs the oracle."""
     from simulation.voxel_mc_3d import run_voxel_mc3d
@@ -195,9 +195,9 @@ def test_degenerate_grids_gpu_vs_oracle(oracle, shape):
 def test_semi_infinite_albedo_sweep_gpu(albedo):
     """V2c (verification_benchmarks.py:343-357): diffuse reflectance of a semi-infinite
     isotropic scatterer (n = 1, mu_t = 10 /mm, normal incidence) against the exact
-    Chandrasekhar H-function value the reference recorded, over 5 x 100 000 photons,
-    with the reference's bar widened from 3 to 4 standard errors (+2e-4 for the
-    30 mm depth truncation)."""
+    Chandrasekhar H-function value the reference recorded, at the reference's bar of 3
+    standard errors (+2e-4 for the 30 mm depth truncation), over 20 x 100 000 photons so
+    that the standard error itself is well estimated; the z-score is printed."""
     import json
     import os
     from simulation.voxel_mc_3d import run_voxel_mc3d
@@ -206,9 +206,11 @@ def test_semi_infinite_albedo_sweep_gpu(albedo):
     mu_t = 10.0
     vals = [run_voxel_mc3d(n_photons=100000, spot_dia=0.01, Lx=60.0, Ly=60.0, Lz=30.0, Nx=6, Ny=6, Nz=30, seed=11 + s,
                            layers_config=[{'z_end': 30.0, 'mu_a': mu_t * (1 - albedo), 'mu_s': mu_t * albedo,
-                                           'g': 0.0, 'n': 1.0}])['spectro_raw']['R_diff'] for s in range(5)]
-    m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(5)
-    assert abs(m - gold["reference_value"]) < 4 * se + 2e-4, (m, se, gold["reference_value"])
+                                           'g': 0.0, 'n': 1.0}])['spectro_raw']['R_diff'] for s in range(20)]
+    m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(len(vals))
+    z = (m - gold["reference_value"]) / se
+    print(f"V2c albedo {albedo}: Rd {m:.6f} exact {gold['reference_value']:.6f} se {se:.2e} z {z:+.2f}")
+    assert abs(m - gold["reference_value"]) < 3 * se + 2e-4, f"albedo {albedo}: z = {z:+.2f} (bar 3 se + 2e-4)"
 
 
 @pytest.mark.parametrize("albedo", [0.5, 0.7, 0.9, 0.99])
@@ -216,7 +218,8 @@ def test_semi_infinite_mismatched_albedo_sweep_gpu(albedo):
     """V2d (verification_benchmarks.py:406-424): total reflectance (specular + diffuse)
     of a semi-infinite isotropic scatterer with n = 1.5 against the reference's
     independent 1-D Monte Carlo (8 x 400 000 photons, its mean and standard error
-    recorded), over 5 x 100 000 photons, within 4 combined standard errors + 2e-4."""
+    recorded), over 20 x 100 000 photons, within the reference's bar of 3 combined standard
+    errors + 2e-4; the z-score is printed."""
     import json
     import os
     from simulation.voxel_mc_3d import run_voxel_mc3d
@@ -224,11 +227,13 @@ def test_semi_infinite_mismatched_albedo_sweep_gpu(albedo):
                                        "reference_engine_values.json")))["values"][f"V2d-a{albedo}"]
     mu_t = 10.0
     vals = []
-    for s in range(5):
+    for s in range(20):
         r = run_voxel_mc3d(n_photons=100000, spot_dia=0.01, Lx=60.0, Ly=60.0, Lz=30.0, Nx=6, Ny=6, Nz=30,
                            seed=31 + s, layers_config=[{'z_end': 30.0, 'mu_a': mu_t * (1 - albedo),
                                                         'mu_s': mu_t * albedo, 'g': 0.0, 'n': 1.5}])
         vals.append(r['spectro_raw']['R_diff'] + r['spectro_raw']['R_spec'])
-    m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(5)
-    bar = 4 * np.hypot(se, gold["series"]["se_ref"]) + 2e-4
-    assert abs(m - gold["reference_value"]) < bar, (m, se, gold["reference_value"])
+    m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(len(vals))
+    sc = np.hypot(se, gold["series"]["se_ref"])
+    z = (m - gold["reference_value"]) / sc
+    print(f"V2d albedo {albedo}: R {m:.6f} reference MC {gold['reference_value']:.6f} combined se {sc:.2e} z {z:+.2f}")
+    assert abs(m - gold["reference_value"]) < 3 * sc + 2e-4, f"albedo {albedo}: z = {z:+.2f} (bar 3 se + 2e-4)"

@@ -128,3 +128,76 @@ def test_arrhenius_with_prefactor_beyond_fp32_matches_oracle():
         assert np.all(np.isfinite(om_gpu))
         np.testing.assert_allclose(om_gpu, om_ref, rtol=2e-4, atol=1e-7)
         assert np.array_equal(om_gpu[abl], om_ref[abl])
+
+
+def _char_inputs(shape, seed):
+    rng = np.random.default_rng(seed)
+    T = 40.0 + 40.0 * rng.random(shape)   # spans the cold cut-off (about 50 C for A = 3.1e98, dt = 50 us)
+    dry = rng.random(shape) < 0.6
+    abl = (rng.random(shape) < 0.1) & ~dry
+    chi = 0.3 * rng.random(shape)
+    return T, dry, abl, chi
+
+
+def test_char_with_prefactor_beyond_fp32_matches_oracle():
+    """_accumulate_char (coupled_ablation.py:43-58) with the skin pre-factor A = 3.1e98 /s (beyond
+    FP32): exp(ln A - Ea/RT) on the device against the FP64 oracle, over dry cells on both sides of
+    the cold cut-off below which rate * dt < 2^-26 leaves chi unchanged in FP32."""
+    from oracle import oracle as O
+    from simulation.coupled_ablation import R_GAS, _accumulate_char
+    shape = (12, 10, 40)
+    T, dry, abl, chi = _char_inputs(shape, 5)
+    A, EaR, dt = 3.1e98, 6.3e5 / R_GAS, 5e-5
+    chi_g = chi.copy()
+    chi_o = chi.astype(np.float32).astype(np.float64)
+    _accumulate_char(T, chi_g, dry, abl, A, EaR, dt, *shape)
+    O.char(T.astype(np.float32).astype(np.float64), chi_o, dry.astype(np.uint8), abl.astype(np.uint8), A, EaR, dt)
+    assert np.all(np.isfinite(chi_g))
+    moved = np.abs(chi_o - chi.astype(np.float32)) > 1e-3
+    assert moved.sum() > 100 and (~moved & dry & ~abl).sum() > 100
+    np.testing.assert_allclose(1.0 - chi_g, 1.0 - chi_o, rtol=2e-4, atol=1e-6)
+    assert np.array_equal(chi_g[~dry | abl], chi.astype(np.float32)[~dry | abl])
+
+
+@pytest.mark.parametrize("pairs", ["0", "1"])
+def test_fused_char_with_prefactor_beyond_fp32_matches_oracle(pairs, monkeypatch):
+    """The fused sub-step's char step with A = 3.1e98 /s, 20 sub-steps, one launch per sub-step
+    (pairs 0) and the two-sub-step kernel (pairs 1, the default on this grid): T within 1e-4 K and
+    1 - chi within 2e-4 relative of the FP64 chain of module kernels."""
+    from oracle import oracle as O
+    from simulation import _bdlib as B
+    from simulation.coupled_ablation import R_GAS
+    from simulation.voxel_thermal_3d import _physics
+    monkeypatch.setenv("BUGDEF_SUBSTEP_PAIRS", pairs)
+    shape = (20, 18, 33)
+    T, dry, abl, chi = _char_inputs(shape, 9)
+    rng = np.random.default_rng(10)
+    k = rng.uniform(0.2e-3, 0.5e-3, shape)
+    rc = rng.uniform(2e-3, 4e-3, shape)
+    Q = rng.uniform(0.0, 50.0, shape)
+    A, EaR, dt, d = 3.1e98, 6.3e5 / R_GAS, 5e-5, 0.015
+    c = B.Context(shape, (d, d, d))
+    c.set_physics(_physics(dt=dt, t_ambient=20.0, t_abl=100.0, t_void=100.0, has_char=1, char_A=A,
+                           char_Ea_over_R=EaR))
+    for name, a in (('T', T), ('Q', Q), ('K', k), ('RHO_CP', rc), ('DRY', dry), ('ABLATED', abl), ('CHI', chi)):
+        c.upload(name, a)
+    c.fill('H_ACC', 0.0)
+    c.substeps(20, 1, B.SUB_PLATEAU | B.SUB_CHAR | B.SUB_MASK_Q)
+    Tg, chig = c.download('T'), c.download('CHI')
+    c.close()
+    f = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    To, Tn, chio, H = f(T), np.empty(shape), f(chi), np.zeros(shape)
+    Qm = f(Q)
+    Qm[abl] = 0.0
+    k, rc = f(k), f(rc)
+    a8, d8 = abl.astype(np.uint8), dry.astype(np.uint8)
+    for _ in range(20):
+        O.thermal_step(To, Tn, Qm, k, rc, *shape, d, d, d, dt, 0.0, 20.0, 1)
+        O.plateau(Tn, H, rc, a8, 100.0, *shape, d8, 100.0)
+        O.char(Tn, chio, d8, a8, A, EaR, dt)
+        To, Tn = Tn, To
+    print("pairs", pairs, "max |dT|", float(np.abs(Tg - To).max()), "max rel d(1-chi)",
+          float((np.abs(chig - chio) / np.maximum(1.0 - chio, 1e-12)).max()))
+    assert float(np.abs(Tg - To).max()) < 1e-4
+    np.testing.assert_allclose(1.0 - chig, 1.0 - chio, rtol=2e-4, atol=1e-6)
+    assert (chio > chi + 0.05).sum() > 100

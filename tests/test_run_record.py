@@ -45,7 +45,7 @@ def test_analyze_npz_closed_forms(tmp_path):
 @pytest.mark.gpu
 def test_recorder_matches_host_slices(tmp_path):
     from simulation.coupled_ablation import CoupledAblationEngine
-    from simulation.run_multispectral_matrix import compute_phase_map
+    from simulation.phase_map import compute_phase_map
     from simulation.run_record import RunRecorder, analyze_npz
     from simulation.workloads import layered_stack
     kw = layered_stack(shape=(64, 48, 80), n_photons=20000, plume_grid=(32, 24, 40))
