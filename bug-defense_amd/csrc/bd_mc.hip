@@ -27,6 +27,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace bd {
 
@@ -554,40 +556,20 @@ __global__ void k_q_from_a(const float2 *__restrict__ AL, float *__restrict__ Q,
         Q[i] = AL[i].x * scale;
 }
 
-// Pack the optics, clear the tallies and fill the kernel arguments of one
-// context's transport (no launch).
-static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a)
+// The beam and grid arguments of one transport launch (voxel_mc_3d.py:599-625): beam frame,
+// incidence vector, launch footprint, Philox key; the buffers are set by the caller.
+static void fill_mc_args(const bd_grid &g, const bd_beam *b, McArgs &a)
 {
-    BD_TRY(launch_pack_optics(c, b->add_plume != 0));
-    BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
-    BD_TRY(ensure(c, (void **)&c->Q, (size_t)c->nvox * sizeof(float)));
-    const int64_t nd = b->launch_axis == 0 ? c->g.nz : c->g.nx;
-    if (c->lat_n != nd) {
-        if (c->lat) BD_HIP(hipFree(c->lat));
-        c->lat = nullptr;
-        BD_HIP(hipMalloc((void **)&c->lat, sizeof(double) * nd));
-        c->lat_n = nd;
-    }
-    BD_TRY(launch_fill(c, c->AL, K_F32, c->nvox * 2, 0.0));
-    BD_TRY(launch_fill(c, c->lat, K_F64, nd, 0.0));
-    BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
-    BD_TRY(launch_fill(c, c->counter, K_F64, 1, 0.0));
-
     a = McArgs{};
-    a.opt = c->optics;
-    a.AL = c->AL;
-    a.acc = c->red + 16;
-    a.lat = c->lat;
-    a.counter = c->counter;
-    a.Nx = c->g.nx;
-    a.Ny = c->g.ny;
-    a.Nz = c->g.nz;
-    a.dx = (float)c->g.dx;
-    a.dy = (float)c->g.dy;
-    a.dz = (float)c->g.dz;
-    a.Lx = (float)(c->g.dx * c->g.nx);
-    a.Ly = (float)(c->g.dy * c->g.ny);
-    a.Lz = (float)(c->g.dz * c->g.nz);
+    a.Nx = g.nx;
+    a.Ny = g.ny;
+    a.Nz = g.nz;
+    a.dx = (float)g.dx;
+    a.dy = (float)g.dy;
+    a.dz = (float)g.dz;
+    a.Lx = (float)(g.dx * g.nx);
+    a.Ly = (float)(g.dy * g.ny);
+    a.Lz = (float)(g.dz * g.nz);
     const double w0 = b->spot_dia / 2.0;
     a.sigma = (float)(w0 / 2.0);
     a.R = (float)w0;
@@ -619,6 +601,33 @@ static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a
     a.k0 = (unsigned)(b->seed & 0xffffffffu);
     a.k1 = (unsigned)(b->seed >> 32);
     a.score_L = b->score_fluence ? 1 : 0;
+}
+
+// Pack the optics, clear the tallies and fill the kernel arguments of one
+// context's transport (no launch).
+static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a)
+{
+    BD_TRY(launch_pack_optics(c, b->add_plume != 0));
+    BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
+    BD_TRY(ensure(c, (void **)&c->Q, (size_t)c->nvox * sizeof(float)));
+    const int64_t nd = b->launch_axis == 0 ? c->g.nz : c->g.nx;
+    if (c->lat_n != nd) {
+        if (c->lat) BD_HIP(hipFree(c->lat));
+        c->lat = nullptr;
+        BD_HIP(hipMalloc((void **)&c->lat, sizeof(double) * nd));
+        c->lat_n = nd;
+    }
+    BD_TRY(launch_fill(c, c->AL, K_F32, c->nvox * 2, 0.0));
+    BD_TRY(launch_fill(c, c->lat, K_F64, nd, 0.0));
+    BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
+    BD_TRY(launch_fill(c, c->counter, K_F64, 1, 0.0));
+
+    fill_mc_args(c->g, b, a);
+    a.opt = c->optics;
+    a.AL = c->AL;
+    a.acc = c->red + 16;
+    a.lat = c->lat;
+    a.counter = c->counter;
     a.ep = ep;
     return BD_OK;
 }
@@ -770,6 +779,114 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
 }
 
 }  // namespace bd
+
+// ------------------------------------------------------------------ allocation-free transport
+// bd_mc_run_noalloc: the transport of bd_mc_run on caller-owned device buffers.  The launch's
+// scratch (photon work counter, FP64 scalar tallies, lateral loss per depth) lives in a
+// file-scope device array instead of library allocations, so the call allocates nothing.
+namespace bd {
+// scratch layout, in doubles: [0] photon work counter (u64), [8, 15) scalar tallies,
+// [16, 16 + depth cells) lateral loss per depth
+constexpr int NOALLOC_ACC = 8, NOALLOC_LAT = 16;
+constexpr int NOALLOC_DOUBLES = NOALLOC_LAT + BD_NOALLOC_MAX_DEPTH;
+}  // namespace bd
+
+static __device__ __align__(256) double scratch[bd::NOALLOC_DOUBLES];
+
+namespace bd {
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double *g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on device `dev`, looked up once per device with that device
+// current.  In host code the array's name is only a stand-in: kernels and copies get this address.
+int scratch_address(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        BD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return BD_OK;
+}
+
+// zero the counter, the scalar tallies and the first n_lat depth bins, and the caller's tallies
+__global__ void k_noalloc_clear(double *__restrict__ s, int n_lat, float *__restrict__ al, int64_t n_al)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (int64_t i = i0; i < NOALLOC_LAT + n_lat; i += stride) s[i] = 0.0;
+    for (int64_t i = i0; i < n_al; i += stride) al[i] = 0.f;
+}
+}  // namespace
+}  // namespace bd
+
+extern "C" int bd_mc_run_noalloc(const bd_grid *grid, const bd_beam *beam, const void *optics, void *AL, void *stream,
+                                 bd_mc_result *out, double *lateral, int64_t n_depth)
+{
+    using namespace bd;
+    BD_CHECK(grid && beam && optics && AL && out, "bd_mc_run_noalloc: null argument");
+    BD_CHECK(grid->nx > 0 && grid->ny > 0 && grid->nz > 0 && grid->dx > 0.0 && grid->dy > 0.0 && grid->dz > 0.0,
+             "bd_mc_run_noalloc: grid dimensions and spacings must be positive");
+    const int64_t nvox = (int64_t)grid->nx * grid->ny * grid->nz;
+    BD_CHECK(nvox < (int64_t(1) << 31), "bd_mc_run_noalloc: at most 2^31 - 1 voxels");
+    BD_CHECK(beam->launch_axis == 0 || beam->launch_axis == 1, "bd_mc_run_noalloc: launch_axis must be 0 or 1");
+    BD_CHECK(beam->beam_profile >= 0 && beam->beam_profile <= 3, "bd_mc_run_noalloc: unknown beam profile");
+    BD_CHECK(beam->n_photons >= 0 && beam->photon_offset >= 0 && beam->max_events > 0,
+             "bd_mc_run_noalloc: photon count, offset and event budget must be non-negative (budget positive)");
+    const int64_t nd = beam->launch_axis == 0 ? grid->nz : grid->nx;
+    if (nd > BD_NOALLOC_MAX_DEPTH) {
+        set_error("bd_mc_run_noalloc: the depth axis has " + std::to_string(nd) +
+                  " cells; the allocation-free transport's scratch holds at most " +
+                  std::to_string(BD_NOALLOC_MAX_DEPTH) + " (use bd_mc_run)");
+        return BD_EINVAL;
+    }
+    BD_CHECK(!lateral || n_depth == nd, "bd_mc_run_noalloc: n_depth must equal the depth-axis cell count");
+    int dev = 0, ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("bd_mc_run_noalloc: no HIP device");
+        return BD_ENODEV;
+    }
+    BD_HIP(hipGetDevice(&dev));
+    BD_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "bd_mc_run_noalloc: device index out of range");
+    double *s = nullptr;
+    BD_TRY(scratch_address(dev, &s));
+    hipStream_t st = (hipStream_t)stream;
+    const int nb_clear = blocks_for(2 * nvox, 256) > 4096 ? 4096 : blocks_for(2 * nvox, 256);
+    hipLaunchKernelGGL(k_noalloc_clear, dim3(nb_clear), dim3(256), 0, st, s, (int)nd, (float *)AL, 2 * nvox);
+    BD_HIP(hipGetLastError());
+    McArgs a;
+    fill_mc_args(*grid, beam, a);
+    a.opt = (const float4 *)optics;
+    a.AL = (float2 *)AL;
+    a.acc = s + NOALLOC_ACC;
+    a.lat = s + NOALLOC_LAT;
+    a.counter = (unsigned long long *)s;
+    a.ep = nullptr;
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t want = (beam->n_photons + 255) / 256;
+    const int64_t cap = (int64_t)ncu * 8;
+    const int nblk = (int)(want < 1 ? 1 : (want < cap ? want : cap));
+    hipLaunchKernelGGL(k_mc, dim3(nblk), dim3(256), 0, st, a);
+    BD_HIP(hipGetLastError());
+    double acc[7];
+    BD_HIP(hipMemcpyAsync(acc, s + NOALLOC_ACC, sizeof(acc), hipMemcpyDeviceToHost, st));
+    if (lateral) BD_HIP(hipMemcpyAsync(lateral, s + NOALLOC_LAT, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    BD_HIP(hipStreamSynchronize(st));
+    out->specular = acc[0];
+    out->diffuse = acc[1];
+    out->transmitted = acc[2];
+    out->lateral = acc[3];
+    out->absorbed = acc[4];
+    out->events = acc[5];
+    out->tally_atomics = acc[6];
+    out->kernel_ms = -1.0;  // not measured: timing events would be allocations
+    return BD_OK;
+}
 
 // ------------------------------------------------------------------ atomic-rate probe
 namespace bd {

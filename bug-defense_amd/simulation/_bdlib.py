@@ -121,6 +121,8 @@ _SIGS = {
     "bd_plume_substeps": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "bd_mc_run": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(MCResult)], ctypes.c_int),
     "bd_mc_lateral": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "bd_mc_run_noalloc": ([ctypes.POINTER(Grid), ctypes.POINTER(Beam), ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.POINTER(MCResult), ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "bd_probe_atomic_rate": ([ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bd_phase_update": ([ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),

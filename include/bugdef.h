@@ -198,6 +198,23 @@ int bd_mc_lateral(bd_ctx *ctx, double *host, int64_t n_depth);
  * launch's device time.  n_pairs is a multiple of 4096 (at most 2^32). */
 int bd_probe_atomic_rate(int device, int64_t n_voxels, int64_t span, int64_t n_pairs, double *kernel_ms);
 
+/* Allocation-free transport.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  The photon transport of bd_mc_run (voxel_mc_3d.py:340-422) on caller-owned
+ * device buffers, with no hipMalloc, hipFree or allocating library call:
+ *   optics  device pointer, Nx*Ny*Nz float4 {mu_a, mu_s, g, n} per voxel, C order;
+ *   AL      device pointer, Nx*Ny*Nz float2 {absorbed weight, path length}, cleared by the call;
+ *   stream  a hipStream_t, or NULL for the null stream; the call waits for it before returning;
+ *   out     the scalar tallies of bd_mc_run (kernel_ms is not measured and reads -1);
+ *   lateral host array of n_depth doubles (n_depth = Nz for launch axis z, Nx for x), or NULL.
+ * Its scratch (photon work counter, FP64 scalar tallies, lateral loss per depth) is a static
+ * device array of 16 + BD_NOALLOC_MAX_DEPTH doubles that each GPU the library runs on holds until
+ * the process ends; a depth axis longer than BD_NOALLOC_MAX_DEPTH cells is rejected (BD_EINVAL).
+ * Calls on one GPU share that array, so calls that overlap in time on one GPU race. */
+#define BD_NOALLOC_MAX_DEPTH 4096
+int bd_mc_run_noalloc(const bd_grid *grid, const bd_beam *beam, const void *optics, void *AL, void *stream,
+                      bd_mc_result *out, double *lateral, int64_t n_depth);
+
 /* Phase update + removal: _update_phase_properties_numba or
  * _update_single_stage_ablation_numba, then _remove with air-gap perforation
  * (coupled_ablation.py:61-180, 551-583, 628-654).  t_now is the engine time
