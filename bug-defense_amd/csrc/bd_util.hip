@@ -388,3 +388,64 @@ void al_set(Ctx *c, const float *s, int which)
 }
 
 }  // namespace bd
+
+// ------------------------------------------------------------------ copy-rate probe
+// The achievable HBM rate the stencils are set against: a streaming float4 copy of one
+// buffer into another (each byte read once and written once), timed with HIP events over
+// one launch after an untimed one.  Grid-stride, 8 blocks of 256 per CU, 16-byte accesses.
+namespace bd {
+__global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                    int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace bd
+
+extern "C" int bd_probe_copy_rate(int device, int64_t bytes, double *kernel_ms)
+{
+    using namespace bd;
+    BD_CHECK(kernel_ms, "bd_probe_copy_rate: null argument");
+    BD_CHECK(bytes >= 16 && bytes % 16 == 0 && bytes <= (int64_t(1) << 36),
+             "bd_probe_copy_rate: bytes must be a positive multiple of 16, at most 64 GiB");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("bd_probe_copy_rate: no HIP device");
+        return BD_ENODEV;
+    }
+    BD_CHECK(device >= 0 && device < ndev, "bd_probe_copy_rate: device index out of range");
+    BD_HIP(hipSetDevice(device));
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e[2] = {nullptr, nullptr};
+    auto run = [&]() -> int {
+        BD_HIP(hipMalloc(&a, (size_t)bytes));
+        BD_HIP(hipMalloc(&b, (size_t)bytes));
+        BD_HIP(hipStreamCreate(&s));
+        BD_HIP(hipEventCreate(&e[0]));
+        BD_HIP(hipEventCreate(&e[1]));
+        const int64_t n = bytes / 16;
+        int ncu = 256;
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+        const int64_t want = (n + 255) / 256;
+        const unsigned nb = (unsigned)(want < (int64_t)ncu * 8 ? want : (int64_t)ncu * 8);
+        hipLaunchKernelGGL(k_copy_probe, dim3(nb), dim3(256), 0, s, a, b, n);  // untimed: first touch
+        BD_HIP(hipGetLastError());
+        BD_HIP(hipEventRecord(e[0], s));
+        hipLaunchKernelGGL(k_copy_probe, dim3(nb), dim3(256), 0, s, a, b, n);
+        BD_HIP(hipGetLastError());
+        BD_HIP(hipEventRecord(e[1], s));
+        BD_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        BD_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+        *kernel_ms = ms;
+        return BD_OK;
+    };
+    const int rc = run();
+    if (e[0]) (void)hipEventDestroy(e[0]);
+    if (e[1]) (void)hipEventDestroy(e[1]);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}

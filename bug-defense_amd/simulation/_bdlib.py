@@ -125,6 +125,7 @@ _SIGS = {
                            ctypes.c_void_p, ctypes.POINTER(MCResult), ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "bd_probe_atomic_rate": ([ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "bd_probe_copy_rate": ([ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "bd_phase_update": ([ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "bd_plume_void": ([ctypes.c_void_p], ctypes.c_int),
     "bd_plume_inject": ([ctypes.c_void_p], ctypes.c_int),
@@ -629,6 +630,15 @@ def scratch_context(shape, spacing) -> Context:
         ctx = Context(*key)
         _cache[key] = ctx
     return ctx
+
+
+def probe_copy_rate(n_bytes=1 << 30, device=0):
+    """HBM bytes moved per second (read + write) by a streaming device copy of n_bytes
+    (bd_probe_copy_rate): the achievable bandwidth the stencils are set against."""
+    n_bytes = int(n_bytes) // 16 * 16
+    ms = ctypes.c_double(0.0)
+    check(lib().bd_probe_copy_rate(int(device), n_bytes, ctypes.byref(ms)), "bd_probe_copy_rate")
+    return 2.0 * n_bytes / (ms.value * 1e-3)
 
 
 def probe_atomic_rate(n_voxels, span=None, n_pairs=1 << 26, device=0):

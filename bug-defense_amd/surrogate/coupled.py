@@ -115,24 +115,43 @@ class SurrogateCoupledEngine:
         self.have_stale = False
         self.judged = False
         self.history = []
+        # per-stage device times of predictions [ms], summed while ``profile`` is set; each stage
+        # ends with a host wait, so profiling costs a few syncs per epoch and stays off by default
+        self.profile = False
+        self.stage_ms = {}
+
+    def _mark(self, name, t0):
+        if not self.profile:
+            return t0
+        self.engine._ctx.sync()
+        self.net.sync()
+        t1 = time.perf_counter()
+        self.stage_ms[name] = self.stage_ms.get(name, 0.0) + (t1 - t0) * 1e3
+        return t1
 
     def _predict(self, beam, into_q):
         ctx = self.engine._ctx
+        t0 = time.perf_counter()
         ws = ctx.sur_windows(beam, self.margin, self.max_windows)
         nb = len(ws)
         mp = self.net.mc_photons
         lo = mp or self.mc_lo
+        t0 = self._mark("windows", t0)
         if lo:
             ctx.mc_run(noisy_beam(beam, lo))
             ctx.sur_keep_stale()
             if self.mc_lo:
                 ctx.sur_keep_aux()
+            t0 = self._mark("mc_lo", t0)
         ctx.sur_features_n(beam, ws, self.x, mp)
         ctx.sync()
+        t0 = self._mark("features", t0)
         self.net.forward(self.x.view(0, (nb, T, T, T, CIN)), self.y.view(0, (nb, T, T, T)))
+        t0 = self._mark("network", t0)
         ctx.sur_apply_n(beam, ws, self.y, into_q, self.scale)
         if self.mc_lo:
             ctx.sur_correct(self.correct_block, into_q)
+        self._mark("compose", t0)
         self.windows = ws
         return ws
 
@@ -188,7 +207,9 @@ class SurrogateCoupledEngine:
         else:
             beam = eng._epoch_beam()
             self._predict(beam, into_q=True)
+            t1 = time.perf_counter()
             d = eng.step_epoch_with_q()
+            self._mark("epoch_without_transport", t1)
             d['optics_source'] = 'surrogate'
             d['judge_error'] = None
             d['mc_vs_mc'] = None
