@@ -355,6 +355,16 @@ static void beam_offsets(const mc_ctx *m, rng_t *r, double *u1, double *u2)
     *u2 = m->p->sigma_axis * rad * sin(2.0 * M_PI * ub);
 }
 
+/* Optional per-voxel counts of the tally additions a device transport that merges consecutive
+ * contributions to one voxel issues: one path-length add per voxel visit, one absorbed-weight add
+ * per visit with a collision.  Off (NULL) unless bdo_mc_set_visit_grids is given two arrays;
+ * used to measure how concentrated the tally traffic is. */
+static double *g_visit_L = 0, *g_visit_A = 0;
+void bdo_mc_set_visit_grids(double *vL, double *vA) { g_visit_L = vL; g_visit_A = vA; }
+#define VISIT_FLUSH() do { if (pv >= 0 && g_visit_L) { if (hL) { _Pragma("omp atomic") g_visit_L[pv] += 1.0; } \
+    if (hA) { _Pragma("omp atomic") g_visit_A[pv] += 1.0; } } pv = -1; hA = hL = 0; } while (0)
+#define VISIT_L(cc) do { if ((cc) != pv) { VISIT_FLUSH(); pv = (cc); } hL = 1; } while (0)
+
 /* voxel_mc_3d.py:123-337 */
 static int64_t trace_one(const mc_ctx *m, rng_t *r, double *A, double *L, double *tally, double *lat)
 {
@@ -406,6 +416,8 @@ static int64_t trace_one(const mc_ctx *m, rng_t *r, double *A, double *L, double
 
     double tau = -log(fmax(rng_u(r), 1e-12));
     int64_t ev;
+    int64_t pv = -1;
+    int hA = 0, hL = 0;
     for (ev = 0; ev < p->max_events; ++ev) {
         const int64_t c = IDX(ic[0], ic[1], ic[2]);
         const double mua = m->mu_a[c];
@@ -428,19 +440,21 @@ static int64_t trace_one(const mc_ctx *m, rng_t *r, double *A, double *L, double
             /* interaction inside the current voxel (:201-265) */
             const double s_col = tau / mut;
             L[c] += w * s_col;
+            VISIT_L(c);
             for (int a = 0; a < 3; ++a) x[a] += s_col * u[a];
             if (m->r_max_sq > 0.0) {
                 const double rr = (la == 0)
                     ? (x[0] - p->x_center) * (x[0] - p->x_center) + (x[1] - p->y_center) * (x[1] - p->y_center)
                     : (x[1] - p->y_center) * (x[1] - p->y_center) + (x[2] - p->z_center) * (x[2] - p->z_center);
-                if (rr >= m->r_max_sq) { tally[3] += w; lat[la == 0 ? ic[2] : ic[0]] += w; return ev + 1; }
+                if (rr >= m->r_max_sq) { tally[3] += w; lat[la == 0 ? ic[2] : ic[0]] += w; VISIT_FLUSH(); return ev + 1; }
             }
             const double dw = w * (mua / mut);
             w -= dw;
             A[c] += dw;
+            if (dw != 0.0) hA = 1;
             if (w < 1e-4) {
                 if (rng_u(r) < 0.1) w *= 10.0;
-                else return ev + 1;
+                else { VISIT_FLUSH(); return ev + 1; }
             }
             const double gl = m->g[c];
             const double xi = rng_u(r);
@@ -473,6 +487,7 @@ static int64_t trace_one(const mc_ctx *m, rng_t *r, double *A, double *L, double
         /* full chord to the exit face (:267-337) */
         tau -= mut * s_exit;
         L[c] += w * s_exit;
+        VISIT_L(c);
         for (int a = 0; a < 3; ++a) x[a] += s_exit * u[a];
         int jc[3] = {ic[0], ic[1], ic[2]};
         jc[axis] += sdir;
@@ -484,17 +499,18 @@ static int64_t trace_one(const mc_ctx *m, rng_t *r, double *A, double *L, double
                 const double r_out = fresnel(m->n[c], p->n_ambient, u[depth_axis]);
                 if (r_out > 0.0 && rng_u(r) < r_out) { u[depth_axis] = -u[depth_axis]; continue; }
                 tally[1] += w;
+                VISIT_FLUSH();
                 return ev + 1;
             }
-            if (jd >= nd[depth_axis]) { tally[2] += w; return ev + 1; }
+            if (jd >= nd[depth_axis]) { tally[2] += w; VISIT_FLUSH(); return ev + 1; }
         } else if (jc[0] < 0 || jc[0] >= Nx || jc[1] < 0 || jc[1] >= Ny || jc[2] < 0 || jc[2] >= Nz) {
-            tally[3] += w; lat[la == 0 ? ic[2] : ic[0]] += w; return ev + 1;
+            tally[3] += w; lat[la == 0 ? ic[2] : ic[0]] += w; VISIT_FLUSH(); return ev + 1;
         }
         if (m->r_max_sq > 0.0) {
             const double rr = (la == 0)
                 ? (x[0] - p->x_center) * (x[0] - p->x_center) + (x[1] - p->y_center) * (x[1] - p->y_center)
                 : (x[1] - p->y_center) * (x[1] - p->y_center) + (x[2] - p->z_center) * (x[2] - p->z_center);
-            if (rr >= m->r_max_sq) { tally[3] += w; lat[la == 0 ? ic[2] : ic[0]] += w; return ev + 1; }
+            if (rr >= m->r_max_sq) { tally[3] += w; lat[la == 0 ? ic[2] : ic[0]] += w; VISIT_FLUSH(); return ev + 1; }
         }
         const double n1 = m->n[c], n2 = m->n[IDX(jc[0], jc[1], jc[2])];
         if (fabs(n1 - n2) > 1e-9) {
@@ -504,6 +520,7 @@ static int64_t trace_one(const mc_ctx *m, rng_t *r, double *A, double *L, double
         }
         ic[0] = jc[0]; ic[1] = jc[1]; ic[2] = jc[2];
     }
+    VISIT_FLUSH();
     return ev;
 }
 

@@ -106,6 +106,8 @@ typedef struct {
  * lat[n_depth] = lateral escape per depth index (Nz for launch_axis 0,
  * Nx for launch_axis 1).  Outputs are ADDED to (caller zeroes them).
  * Returns the total number of transport events traced. */
+/* optional per-voxel counts of merged tally additions (path-length, absorbed); NULL disables */
+void bdo_mc_set_visit_grids(double *visit_L, double *visit_A);
 int64_t bdo_mc_trace(const bdo_mc_params *p, const double *mu_a,
                      const double *mu_s, const double *g, const double *n,
                      double *A, double *L, double *tally, double *lat);

@@ -110,6 +110,7 @@ def lib():
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.bdo_mc_trace.argtypes = [ctypes.POINTER(_MCParams), _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.bdo_mc_trace.restype = ctypes.c_int64
+        L.bdo_mc_set_visit_grids.argtypes = [_dp, _dp]
         L.bdo_u01.argtypes = [ctypes.c_uint32]
         L.bdo_u01.restype = ctypes.c_double
         _lib = L
@@ -275,8 +276,13 @@ def run_voxel_mc3d(power=1.0, pulse_duration=0.20, n_photons=200000, spot_dia=0.
                    phi_inc_deg=0.0, Lx=4.0, Ly=4.0, Lz=2.42, Nx=80, Ny=80, Nz=60, r_max=0.0,
                    layers_config=None, custom_3d_props=None, launch_axis='z', beam_center=None,
                    beam_profile='gaussian', seed=0, photon_offset=0, n_threads=None,
-                   max_events=MAX_EVENTS_PER_PHOTON):
-    """voxel_mc_3d.py:425-717 on the CPU (FP64), Philox stream ``seed``."""
+                   max_events=MAX_EVENTS_PER_PHOTON, visit_grids=None):
+    """voxel_mc_3d.py:425-717 on the CPU (FP64), Philox stream ``seed``.
+
+    visit_grids: optional pair of float64 (Nx, Ny, Nz) arrays to which the per-voxel counts of
+    merged path-length and absorbed-weight tally additions are added (one per voxel visit, and
+    one per visit with a collision): the additions a device transport that sums consecutive
+    contributions to one voxel before issuing them would make."""
     dx, dy, dz = Lx / Nx, Ly / Ny, Lz / Nz
     la = 1 if launch_axis.strip().lower() == 'x' else 0
     if custom_3d_props is not None:
@@ -315,9 +321,17 @@ def run_voxel_mc3d(power=1.0, pulse_duration=0.20, n_photons=200000, spot_dia=0.
     L = np.zeros((Nx, Ny, Nz))
     tally = np.zeros(4)
     lat = np.zeros(Nz if la == 0 else Nx)
+    if visit_grids is not None:
+        vL, vA = visit_grids
+        assert vL.shape == vA.shape == (Nx, Ny, Nz)
+        lib().bdo_mc_set_visit_grids(_d(vL), _d(vA))
     t0 = time.perf_counter()
-    events = lib().bdo_mc_trace(ctypes.byref(p), _d(mu_a), _d(mu_s), _d(g), _d(n), _d(A), _d(L), _d(tally),
-                                _d(lat))
+    try:
+        events = lib().bdo_mc_trace(ctypes.byref(p), _d(mu_a), _d(mu_s), _d(g), _d(n), _d(A), _d(L), _d(tally),
+                                    _d(lat))
+    finally:
+        if visit_grids is not None:
+            lib().bdo_mc_set_visit_grids(None, None)
     elapsed = time.perf_counter() - t0
     N = float(n_photons)
     R_spec, R_diff, T_tot, Lat = (float(v) / N for v in tally)

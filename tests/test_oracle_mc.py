@@ -183,3 +183,69 @@ def test_launch_axis_x_matches_z_statistics(oracle):
     for key in ('R_diff', 'Absorption', 'Transmission'):
         assert abs(rz['spectro_raw'][key] - rx['spectro_raw'][key]) < 0.006
     assert rz['spectro_raw']['R_spec'] == pytest.approx(rx['spectro_raw']['R_spec'], abs=1e-12)
+
+
+def test_visit_counts_closed_form(oracle):
+    """The merged-tally visit counts: through a transparent, index-matched column every photon
+    visits every voxel once (path-length adds only) and no collision happens."""
+    shape = (1, 1, 12)
+    vL, vA = np.zeros(shape), np.zeros(shape)
+    props = {'mu_a': np.zeros(shape), 'mu_s': np.zeros(shape), 'g': np.zeros(shape), 'n': np.ones(shape)}
+    r = oracle.run_voxel_mc3d(n_photons=5000, spot_dia=1e-4, Lx=0.1, Ly=0.1, Lz=1.2, Nx=1, Ny=1, Nz=12,
+                              custom_3d_props=props, seed=2, n_threads=4, visit_grids=(vL, vA))
+    assert r['spectro_raw']['Transmission'] == pytest.approx(1.0)
+    np.testing.assert_array_equal(vL, np.full(shape, 5000.0))
+    assert not vA.any()
+
+
+def test_tally_concentration_on_the_bench_state(oracle):
+    """How concentrated the transport's tally additions are on the 256x256x192 layered stack after
+    three epochs (150 000 photons): the share held by the densest voxels and by the best dense box
+    a per-block window in LDS could index, and the hits per voxel one 256-photon block makes there.
+    Measured: about 163 merged additions per photon; the densest 20 480 voxels (160 KB of float2
+    tallies) take 34 % of them, the best box of at most 20 480 voxels 27 %, and one block of 256
+    photons adds about 0.7 times per voxel there, so a block-private window would flush about as
+    many additions as it saves.  The bounds below keep that finding true."""
+    from simulation.workloads import layered_stack
+    kw = layered_stack()
+    okw = dict(kw)
+    pg = okw.pop('plume_grid')
+    eng = oracle.OracleEngine(mc_threads=8, plume_grid=pg, **okw)
+    for _ in range(3):
+        eng.step_epoch()
+    opt = {k: v.copy() for k, v in eng.optical.items()}
+    opt['mu_a'] = eng.optical['mu_a'] + eng._from_plume(eng.plume.mu_a_plume)
+    opt['mu_s'] = eng.optical['mu_s'] + eng._from_plume(eng.plume.mu_s_plume)
+    opt['g'][(eng._from_plume(eng.plume.mu_ext_plume) > 0.001) & eng.ablated] = 0.80
+    shape = eng.T.shape
+    vL, vA = np.zeros(shape), np.zeros(shape)
+    N = 150_000
+    oracle.run_voxel_mc3d(power=kw['power'], n_photons=N, spot_dia=kw['spot_dia'], theta_inc_deg=kw['theta_inc_deg'],
+                          Lx=kw['Lx'], Ly=kw['Ly'], Lz=kw['Lz'], Nx=shape[0], Ny=shape[1], Nz=shape[2],
+                          custom_3d_props=opt, beam_center=kw['beam_center'], seed=12345, n_threads=8,
+                          visit_grids=(vL, vA))
+    v = vL + vA
+    tot = float(v.sum())
+    cs = np.cumsum(np.sort(v.ravel())[::-1])
+    blocks = N / 256.0
+    print(f"merged tally additions per photon {tot / N:.1f} (path {vL.sum() / N:.1f}, absorbed {vA.sum() / N:.1f})")
+    for k in (4096, 10240, 20480, 65536):
+        print(f"densest {k} voxels: share {cs[k - 1] / tot:.3f}, hits per voxel per 256-photon block "
+              f"{cs[k - 1] / k / blocks:.2f}")
+    S = np.zeros(tuple(n + 1 for n in shape))
+    S[1:, 1:, 1:] = v.cumsum(0).cumsum(1).cumsum(2)
+    best = (0.0, None)
+    for bz in (8, 12, 16, 20, 24, 32, 48, 64):
+        for b in (12, 16, 20, 24, 32, 48):
+            if b * b * bz > 20480:
+                continue
+            box = (S[b:, b:, bz:] - S[:-b, b:, bz:] - S[b:, :-b, bz:] - S[b:, b:, :-bz] + S[:-b, :-b, bz:]
+                   + S[:-b, b:, :-bz] + S[b:, :-b, :-bz] - S[:-b, :-b, :-bz])
+            m = float(box.max())
+            if m > best[0]:
+                best = (m, (b, b, bz))
+    print(f"best box of <= 20480 voxels {best[1]}: share {best[0] / tot:.3f}, hits per voxel per block "
+          f"{best[0] / np.prod(best[1]) / blocks:.2f}")
+    assert 100 < tot / N < 250
+    assert cs[20479] / tot < 0.45 and best[0] / tot < 0.35
+    assert cs[20479] / 20480 / blocks < 1.0
