@@ -354,6 +354,65 @@ def run_voxel_mc3d(power=1.0, pulse_duration=0.20, n_photons=200000, spot_dia=0.
     }
 
 
+# --------------------------------------------------------------------------- stand-alone thermal solver
+class OracleThermalSolver3D:
+    """voxel_thermal_3d.py:146-309 (VoxelThermalSolver3D) in FP64 on the CPU."""
+
+    CFL_SAFETY_FACTOR = 0.40
+
+    def __init__(self, grid_x, grid_y, grid_z, k_grid, rho_cp_grid, h_conv=15.0, emissivity=0.95, t_ambient=20.0):
+        self.Nx, self.Ny, self.Nz = len(grid_x), len(grid_y), len(grid_z)
+        self.dx = float(grid_x[1] - grid_x[0]) if self.Nx > 1 else 0.05
+        self.dy = float(grid_y[1] - grid_y[0]) if self.Ny > 1 else 0.05
+        self.dz = float(grid_z[1] - grid_z[0]) if self.Nz > 1 else 0.04
+        k = np.asarray(k_grid, dtype=np.float64)
+        rc = np.asarray(rho_cp_grid, dtype=np.float64)
+        # :171-180: W/(m K) -> W/(mm K) when k > 0.05; J/(m^3 K) -> J/(mm^3 K) when rho_cp > 100
+        self.k_grid = k * 1e-3 if np.max(k) > 0.05 else k.copy()
+        self.rho_cp_grid = rc * 1e-9 if np.max(rc) > 100.0 else rc.copy()
+        # :185-188: convection plus linearised radiation, W/(m^2 K) -> W/(mm^2 K)
+        self.h_eff = (float(h_conv) + 4.0 * 5.670374e-8 * float(emissivity) * (t_ambient + 273.15) ** 3) * 1e-6
+        self.t_ambient = float(t_ambient)
+
+    def cfl_conditioned_rho_cp(self, dt):
+        """:191-203."""
+        inv_d2 = 1.0 / self.dx ** 2 + 1.0 / self.dy ** 2 + 1.0 / self.dz ** 2
+        amax = self.CFL_SAFETY_FACTOR / (dt * inv_d2)
+        rc = self.rho_cp_grid.copy()
+        unstable = self.k_grid / np.maximum(rc, 1e-12) > amax
+        if np.any(unstable):
+            rc[unstable] = self.k_grid[unstable] / amax
+        return rc
+
+    def solve(self, Q_3d, pulse_duration=0.20, total_time=1.00, dt=0.001, snapshot_times=None):
+        """:205-309: explicit steps, source on while t < pulse_duration, centre and surface
+        histories at every step (step 0 included), snapshots at the rounded target times."""
+        n_steps = int(round(total_time / dt))
+        rc = self.cfl_conditioned_rho_cp(dt)
+        Q = np.ascontiguousarray(Q_3d, dtype=np.float64)
+        T = np.full((self.Nx, self.Ny, self.Nz), self.t_ambient)
+        Tn = np.empty_like(T)
+        if snapshot_times is None:
+            snapshot_times = [0.0, pulse_duration, total_time]
+        targets = sorted({round(float(t), 4) for t in snapshot_times})
+        mx, my, mz = self.Nx // 2, self.Ny // 2, self.Nz // 2
+        centre, surface, times, snaps = [], [], [], {}
+        for step in range(n_steps + 1):
+            cur = round(step * dt, 4)
+            centre.append(float(T[mx, my, mz]))
+            surface.append(float(T[mx, my, 0]))
+            times.append(cur)
+            if cur in targets:
+                snaps[cur] = T.copy()
+            if step < n_steps:
+                thermal_step(T, Tn, Q, self.k_grid, rc, self.Nx, self.Ny, self.Nz, self.dx, self.dy, self.dz, dt,
+                             self.h_eff, self.t_ambient, cur < pulse_duration)
+                T, Tn = Tn, T
+        return {'snapshots': snaps, 'time_history': np.array(times), 'center_history': np.array(centre),
+                'surface_history': np.array(surface), 'peak_surface_T': max(surface), 'peak_center_T': max(centre),
+                'rho_cp_eff': rc, 'T_final': T}
+
+
 # --------------------------------------------------------------------------- plume
 class OraclePlume:
     """plume.py:142-438 (MultispectralPlumeField) restated."""

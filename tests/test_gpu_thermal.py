@@ -283,3 +283,45 @@ def test_substep_tile_shape_does_not_change_the_result(monkeypatch):
     for o in out[1:]:
         for n in out[0]:
             np.testing.assert_array_equal(out[0][n], o[n], err_msg=n)
+
+
+@pytest.mark.parametrize("case", ["reference_fixture", "si_units_with_air"])
+def test_voxel_thermal_solver_histories_match_oracle(oracle, case):
+    """VoxelThermalSolver3D (voxel_thermal_3d.py:146-309) on the device against the FP64 oracle
+    solver: every entry of the centre and surface histories and every snapshot within 1e-4 K.
+    The first case is the reference's own fixture (tests/test_solvers.py:41-75); the second passes
+    k and rho c in SI units (the unit heuristics) with an air layer that the CFL raise conditions."""
+    from simulation.voxel_thermal_3d import VoxelThermalSolver3D
+    if case == "reference_fixture":
+        n = (8, 8, 6)
+        grids = (np.linspace(-0.5, 0.5, 8), np.linspace(-0.5, 0.5, 8), np.linspace(0.0, 0.6, 6))
+        k, rc = np.full(n, 0.2e-3), np.full(n, 1.8e-3)
+        Q = np.zeros(n)
+        Q[3:5, 3:5, 0:2] = 20.0
+        kw = dict(h_conv=15.0, emissivity=0.95, t_ambient=25.0)
+        run = dict(pulse_duration=0.005, total_time=0.015, dt=0.001)
+    else:
+        n = (24, 20, 30)
+        grids = (np.arange(24) * 0.05, np.arange(20) * 0.05, np.arange(30) * 0.05)
+        k, rc = np.full(n, 0.5), np.full(n, 4.0e6)          # W/(m K), J/(m^3 K)
+        k[:, :, :4], rc[:, :, :4] = 0.026, 1206.0            # air on top
+        Q = np.zeros(n)
+        Q[9:15, 8:12, 4:12] = 10.0
+        kw = dict(h_conv=25.0, emissivity=0.9, t_ambient=20.0)
+        run = dict(pulse_duration=0.03, total_time=0.06, dt=0.001, snapshot_times=[0.0, 0.01, 0.03, 0.045])
+    g = VoxelThermalSolver3D(*grids, k_grid=k, rho_cp_grid=rc, **kw).solve(Q_3d=Q, **run)
+    o_solver = oracle.OracleThermalSolver3D(*grids, k_grid=k, rho_cp_grid=rc, **kw)
+    r = o_solver.solve(Q, **run)
+    if case != "reference_fixture":
+        assert np.any(o_solver.cfl_conditioned_rho_cp(run['dt']) != o_solver.rho_cp_grid)
+    np.testing.assert_array_equal(g['time_history'], r['time_history'])
+    for key in ('center_history', 'surface_history'):
+        d = float(np.max(np.abs(g[key] - r[key])))
+        print(case, key, "max |dT|", d)
+        assert d < 1e-4, key
+    assert sorted(g['snapshots']) == sorted(r['snapshots'])
+    for t in r['snapshots']:
+        assert float(np.max(np.abs(g['snapshots'][t] - r['snapshots'][t]))) < 1e-4, t
+    assert abs(g['peak_surface_T'] - r['peak_surface_T']) < 1e-4
+    assert abs(g['peak_center_T'] - r['peak_center_T']) < 1e-4
+    assert max(r['peak_surface_T'], r['peak_center_T']) - kw['t_ambient'] > 1.0

@@ -224,3 +224,78 @@ def test_arrhenius_closed_form(oracle):
     exp_om = 100 * 1e-4 * 3.1e98 * np.exp(-6.28e5 / 8.314 / 343.15)
     np.testing.assert_allclose(Om[1:], exp_om, rtol=1e-12)
     assert np.all(Om[0] == 0.0)
+
+
+# ------------------------------------------------------------------ VoxelThermalSolver3D (voxel_thermal_3d.py:146-309)
+def _solver_grids(n=(8, 8, 6), span=(1.0, 1.0, 0.6)):
+    return (np.linspace(-span[0] / 2, span[0] / 2, n[0]), np.linspace(-span[1] / 2, span[1] / 2, n[1]),
+            np.linspace(0.0, span[2], n[2]))
+
+
+def test_oracle_solver_unit_heuristics_and_boundary(oracle):
+    """:171-188: k above 0.05 is taken as W/(m K), rho c above 100 as J/(m^3 K); h_eff adds the
+    linearised radiation 4 sigma eps T_amb^3.  For h_conv 10 W/(m^2 K), eps 0.9, 22 C:
+    4 x 5.670374e-8 x 0.9 x 295.15^3 = 5.24859 W/(m^2 K), so h_eff = 15.24859e-6 W/(mm^2 K)."""
+    gx, gy, gz = _solver_grids()
+    shape = (8, 8, 6)
+    s_si = oracle.OracleThermalSolver3D(gx, gy, gz, np.full(shape, 0.5), np.full(shape, 4.0e6), h_conv=10.0,
+                                        emissivity=0.9, t_ambient=22.0)
+    s_mm = oracle.OracleThermalSolver3D(gx, gy, gz, np.full(shape, 0.5e-3), np.full(shape, 4.0e-3), h_conv=10.0,
+                                        emissivity=0.9, t_ambient=22.0)
+    np.testing.assert_allclose(s_si.k_grid, s_mm.k_grid, rtol=1e-15)
+    np.testing.assert_allclose(s_si.rho_cp_grid, s_mm.rho_cp_grid, rtol=1e-15)
+    assert s_mm.k_grid[0, 0, 0] == 0.5e-3 and s_mm.rho_cp_grid[0, 0, 0] == 4.0e-3
+    assert s_si.h_eff == pytest.approx(15.24859e-6, rel=1e-6)
+    assert s_si.dx == pytest.approx(1.0 / 7) and s_si.dz == pytest.approx(0.12)
+
+
+def test_oracle_solver_cfl_raise(oracle):
+    """:191-203: after the raise every cell satisfies k / rho_c <= 0.40 / (dt sum 1/d^2), raised cells
+    sit exactly on the limit and stable cells keep their rho_c."""
+    gx, gy, gz = _solver_grids()
+    rng = np.random.default_rng(4)
+    shape = (8, 8, 6)
+    k = rng.uniform(0.05e-3, 0.6e-3, shape)
+    k[:, :, :2] = 0.026e-3        # air-like cells: tiny rho c, large diffusivity
+    rc = rng.uniform(2e-3, 4e-3, shape)
+    rc[:, :, :2] = 1.2e-6
+    s = oracle.OracleThermalSolver3D(gx, gy, gz, k, rc, h_conv=0.0, emissivity=0.0)
+    dt = 1e-3
+    amax = 0.40 / (dt * (1 / s.dx ** 2 + 1 / s.dy ** 2 + 1 / s.dz ** 2))
+    rce = s.cfl_conditioned_rho_cp(dt)
+    raised = rce != rc
+    assert raised[:, :, :2].all() and not raised[:, :, 2:].any()
+    np.testing.assert_allclose((k / rce)[raised], amax, rtol=1e-14)
+    assert np.all(k / rce <= amax * (1 + 1e-14))
+
+
+def test_oracle_solver_conserves_energy_insulated(oracle):
+    """With h_conv = 0 and emissivity 0 the boundary is insulated, so the heat stored at the end
+    equals the source energy sum Q V dt over the steps the source was on (t < pulse_duration)."""
+    gx, gy, gz = _solver_grids()
+    shape = (8, 8, 6)
+    s = oracle.OracleThermalSolver3D(gx, gy, gz, np.full(shape, 0.2e-3), np.full(shape, 1.8e-3), h_conv=0.0,
+                                     emissivity=0.0, t_ambient=25.0)
+    Q = np.zeros(shape)
+    Q[3:5, 3:5, 0:2] = 20.0
+    res = s.solve(Q, pulse_duration=0.005, total_time=0.015, dt=0.001)
+    V = s.dx * s.dy * s.dz
+    stored = float(np.sum(res['rho_cp_eff'] * (res['T_final'] - 25.0))) * V
+    on_steps = sum(1 for t in res['time_history'][:-1] if t < 0.005)
+    assert on_steps == 5
+    assert stored == pytest.approx(float(Q.sum()) * V * 0.001 * on_steps, rel=1e-12)
+    # histories: one entry per step plus the initial state, snapshots at the rounded targets
+    assert len(res['time_history']) == 16 and sorted(res['snapshots']) == [0.0, 0.005, 0.015]
+    assert res['center_history'][0] == 25.0 and res['peak_surface_T'] >= 25.0
+
+
+def test_oracle_solver_reference_fixture(oracle):
+    """tests/test_solvers.py:41-75 of the reference on the oracle solver."""
+    gx, gy, gz = (np.linspace(-0.5, 0.5, 8), np.linspace(-0.5, 0.5, 8), np.linspace(0.0, 0.6, 6))
+    s = oracle.OracleThermalSolver3D(gx, gy, gz, np.full((8, 8, 6), 0.2e-3), np.full((8, 8, 6), 1.8e-3),
+                                     h_conv=15.0, emissivity=0.95, t_ambient=25.0)
+    Q = np.zeros((8, 8, 6))
+    Q[3:5, 3:5, 0:2] = 20.0
+    res = s.solve(Q, pulse_duration=0.005, total_time=0.015, dt=0.001)
+    assert res['peak_surface_T'] >= 25.0 and not np.any(np.isnan(res['time_history']))
+    assert len(res['time_history']) > 0 and res['peak_surface_T'] > 25.01
