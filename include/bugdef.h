@@ -199,9 +199,9 @@ int bd_mc_lateral(bd_ctx *ctx, double *host, int64_t n_depth);
 int bd_probe_atomic_rate(int device, int64_t n_voxels, int64_t span, int64_t n_pairs, double *kernel_ms);
 
 /* Measurement helper, no counterpart in the reference: the achievable HBM rate the stencils
- * are set against.  One launch of a streaming float4 copy of `bytes` (a multiple of 16) from one
- * device buffer into another on device `device`, after an untimed launch; writes the launch's
- * device time.  The copy moves 2 * bytes through HBM. */
+ * are set against.  A streaming float4 copy of `bytes` (a multiple of 16) from one device buffer
+ * into another on device `device`, timed in six launch shapes (each after an untimed launch);
+ * writes the fastest launch's device time.  The copy moves 2 * bytes through HBM. */
 int bd_probe_copy_rate(int device, int64_t bytes, double *kernel_ms);
 
 /* Allocation-free transport.  No counterpart in the reference: it is for callers that cannot

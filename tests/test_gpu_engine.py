@@ -440,5 +440,6 @@ def test_mcmillan_pixel_depths_match_reference_engine():
             eng.close()
     print({E: (round(got[E], 4), ref[f"M-{E}-exp"]["model"]) for E in got})
     for E, d in got.items():
-        # within two voxels of the reference engine's depth
-        assert abs(d - ref[f"M-{E}-exp"]["model"]) <= 2 * voxel + 1e-9, E
+        # within one voxel of the reference engine's depth (the runs so far reproduce it exactly;
+        # one voxel leaves room for the unseeded reference transport)
+        assert abs(d - ref[f"M-{E}-exp"]["model"]) <= voxel + 1e-9, E
