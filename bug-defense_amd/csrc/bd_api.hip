@@ -143,8 +143,15 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     BD_HIP(hipEventCreate(&c->ev3));
     BD_HIP(hipEventCreate(&c->ev4));
     BD_HIP(hipEventCreate(&c->ev5));
-    for (int i = 0; i < 6; ++i) BD_HIP(hipEventCreate(&c->gev[i]));
+    BD_HIP(hipEventCreate(&c->ev6));
+    BD_HIP(hipEventCreate(&c->ev7));
+    BD_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    BD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    BD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    for (int i = 0; i < 7; ++i) BD_HIP(hipEventCreate(&c->gev[i]));
     if (const char *e = std::getenv("BUGDEF_MC_BLOCKS_PER_CU")) c->mc_blocks_per_cu = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("BUGDEF_MC_DEFER_FLUENCE")) c->defer_L = std::atoi(e) != 0;
+    if (const char *e = std::getenv("BUGDEF_MC_RECORDS_PER_PHOTON")) c->rec_per_photon = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("BUGDEF_SUBSTEP_PAIRS")) c->two_step_allowed = std::atoi(e) != 0;
     // a fixed one-step tile "BYxXC" (BY 2, 4 or 8; XC >= 1) in place of the timed choice, e.g. for
     // counter runs, where the profiler's per-dispatch overhead distorts the tuning launches
@@ -167,12 +174,14 @@ int bd_destroy(bd_ctx *h)
 {
     if (!h) return BD_OK;
     Ctx *c = &h->c;
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->T, c->T2, c->H, c->Om, c->chi, c->Q, c->k, c->rc, c->hlat, c->flags, c->mu_a, c->mu_s,
                     c->gg, c->nn, c->wet_mu_a, c->wet_k, c->wet_rc, c->Tpeak, c->abl_time, c->AL, c->optics,
                     c->P, c->P2, c->Pext, c->pvoid, c->to_remove, c->red, c->lat, c->counter, c->staging,
                     c->splat_tmp1, c->splat_tmp2, c->kb[0], c->kb[1], c->kb[2], c->kl[0], c->kl[1], c->kl[2],
-                    c->Qs, c->Qsur, c->Qaux, c->Hb, c->flagsb};
+                    c->Qs, c->Qsur, c->Qaux, c->Hb, c->flagsb, c->rec, c->rec_sorted, c->rec_tcnt,
+                    c->rec_tstart, c->rec_off, c->rec_cstart, c->rec_cmap};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &f : c->ds)
@@ -189,7 +198,12 @@ int bd_destroy(bd_ctx *h)
     if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->ev4) (void)hipEventDestroy(c->ev4);
     if (c->ev5) (void)hipEventDestroy(c->ev5);
-    for (int i = 0; i < 6; ++i)
+    if (c->ev6) (void)hipEventDestroy(c->ev6);
+    if (c->ev7) (void)hipEventDestroy(c->ev7);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    for (int i = 0; i < 7; ++i)
         if (c->gev[i]) (void)hipEventDestroy(c->gev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete h;
@@ -344,6 +358,7 @@ int bd_upload(bd_ctx *h, int field, int dtype, const void *host, int64_t n)
         break;
     }
     case KAL: {
+        BD_TRY(join_records(c));
         BD_TRY(ensure(c, (void **)&c->AL, (size_t)n * sizeof(float2)));
         BD_TRY(ensure_staging(c, (size_t)n * 12));
         float *tmp = (float *)((char *)c->staging + (size_t)n * 8);
@@ -406,6 +421,7 @@ int bd_download(bd_ctx *h, int field, int dtype, void *host, int64_t n)
         BD_HIP(hipMemcpyAsync(host, c->staging, (size_t)n, hipMemcpyDeviceToHost, c->stream));
         break;
     case KAL: {
+        BD_TRY(join_records(c));
         BD_TRY(ensure_staging(c, (size_t)n * 12));
         float *tmp = (float *)((char *)c->staging + (size_t)n * 8);
         al_get(c, tmp, r.which);
@@ -697,6 +713,7 @@ int bd_reduce(bd_ctx *h, int field, int op, int mask_field, double *out)
         bits = m.kind == KFLAG ? m.bit : 0xFF;
     }
     if (r.kind == KAL) {
+        BD_TRY(join_records(c));
         BD_TRY(ensure_staging(c, (size_t)c->nvox * 4));
         al_get(c, (float *)c->staging, r.which);
         return launch_reduce(c, c->staging, K_F32, r.n, op, mask, bits, out);

@@ -68,7 +68,7 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         out->mc_energy_closure = std::nan("");
     } else if (beam->power > 0.0) {
         bd_mc_result mc{};
-        BD_TRY(launch_mc(c, beam, &mc));
+        BD_TRY(launch_mc(c, beam, &mc, true));  // record passes beside the sub-steps, joined below
         const double N = (double)beam->n_photons;
         out->absorbed_frac = mc.absorbed / N;
         out->mc_energy_closure = (mc.specular + mc.diffuse + mc.transmitted + mc.lateral + mc.absorbed) / N;
@@ -81,11 +81,14 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         out->lateral_loss = mc.lateral / N;
         out->mc_events = mc.events;
         out->mc_tally_atomics = mc.tally_atomics;
+        out->mc_tally_records = mc.tally_records;
         out->t_mc_kernel_ms = mc.kernel_ms;
+        out->t_mc_records_ms = mc.record_ms;  // NaN while they run beside the sub-steps
         double qsum = 0.0;
         BD_TRY(reduce_masked(c, c->Q, c->nvox, c->flags, F_ABL, true, &qsum));
         out->absorbed_J = qsum * V * dt_epoch;
     } else {
+        BD_TRY(join_records(c));
         BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
         if (c->AL) BD_TRY(launch_fill(c, c->AL, K_F32, 2 * c->nvox, 0.0));
         out->absorbed_frac = 0.0;
@@ -133,11 +136,17 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         out->plume_tau = 0.0;
         out->plume_transmission = 1.0;
     }
+    const bool records_beside = c->join_pending;
+    BD_TRY(join_records(c));
     BD_HIP(hipEventRecord(c->ev1, c->stream));
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     out->conducted_J = -*(double *)c->host_pinned;
     float ms = 0.f;
+    if (records_beside) {
+        BD_HIP(hipEventElapsedTime(&ms, c->ev7, c->ev6));
+        out->t_mc_records_ms = ms;
+    }
     BD_HIP(hipEventElapsedTime(&ms, e_start, e_opt));
     out->t_optical_ms = ms;
     BD_HIP(hipEventElapsedTime(&ms, e_opt, c->ev0));
@@ -166,6 +175,7 @@ static int enqueue_after_mc(Ctx *c, const bd_beam *beam)
     if (beam->power > 0.0) {
         BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
     } else {
+        BD_TRY(join_records(c));
         BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
         if (c->AL) BD_TRY(launch_fill(c, c->AL, K_F32, 2 * c->nvox, 0.0));
         BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
@@ -202,6 +212,7 @@ static int enqueue_epoch_tail(Ctx *c, int n_substeps)
         BD_TRY(launch_plume_sync(c));
         BD_TRY(launch_plume_diag_enqueue(c));
     }
+    BD_TRY(join_records(c));  // the record passes ran beside the sub-steps
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red, 48 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipMemcpyAsync((char *)c->host_pinned + 48 * sizeof(double), c->counter, 2 * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, c->stream));
@@ -210,7 +221,7 @@ static int enqueue_epoch_tail(Ctx *c, int n_substeps)
 
 static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
 {
-    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2));
+    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2, true));
     BD_TRY(enqueue_after_mc(c, beam));
     return enqueue_epoch_tail(c, n_substeps);
 }
@@ -293,6 +304,7 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
         BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
         BD_TRY(ensure(c, (void **)&c->optics, (size_t)c->nvox * sizeof(float4)));
         BD_TRY(ensure(c, (void **)&c->hlat, (size_t)c->nvox * 4));
+        BD_TRY(ensure_records(c, beam));
         const int64_t nd = beam->launch_axis == 0 ? c->g.nz : c->g.nx;
         if (c->lat_n != nd) {
             if (c->lat) BD_HIP(hipFree(c->lat));
@@ -400,6 +412,7 @@ extern "C" int bd_epoch_graph_finish(bd_ctx *h, bd_epoch_diag *out)
         out->lateral_loss = m[3] / N;
         out->mc_events = m[5];
         out->mc_tally_atomics = m[6];
+        out->mc_tally_records = m[7];
         out->absorbed_J = r[40] * V * dt_epoch;
     } else {
         out->mc_energy_closure = 1.0;
@@ -430,6 +443,8 @@ extern "C" int bd_epoch_graph_finish(bd_ctx *h, bd_epoch_diag *out)
     if (beam->power > 0.0) {
         BD_HIP(hipEventElapsedTime(&ms, c->gev[0], c->gev[1]));
         out->t_mc_kernel_ms = ms;
+        BD_HIP(hipEventElapsedTime(&ms, c->gev[1], c->gev[6]));
+        out->t_mc_records_ms = ms;
     }
     return BD_OK;
 }

@@ -50,9 +50,15 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
+    hipEvent_t ev6 = nullptr, ev7 = nullptr;  // end / start of the path-length record passes
+    // the record passes of an epoch's transport run on `side`, beside the thermal sub-steps
+    // (nothing in the epoch reads the path-length tally); join_records() makes `stream` wait
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool join_pending = false;
     // recorded as external event nodes inside the captured epoch graph: MC kernel start/end,
-    // optics end, sub-steps start/end, epoch end
-    hipEvent_t gev[6] = {};
+    // optics end, sub-steps start/end, epoch end, end of the path-length record pass
+    hipEvent_t gev[7] = {};
     bd_physics phys{};
     bool has_phys = false;
     bd_plume_params plume{};
@@ -88,7 +94,14 @@ struct Ctx {
     double *red = nullptr;               // reduction accumulators (64 doubles) + 8192 doubles of partials
     double *lat = nullptr;               // lateral escape per depth
     int64_t lat_n = 0;
-    unsigned long long *counter = nullptr;  // photon work counter + misc ints
+    unsigned long long *counter = nullptr;  // [0] photon work counter, [1] n_new, [2] breach, [3] record cursor
+    uint2 *rec = nullptr;                // deferred path-length records of the transport
+    uint2 *rec_sorted = nullptr;         // the records in tile order (record passes)
+    int64_t rec_cap = 0;                 // records each of the two buffers holds
+    unsigned *rec_tcnt = nullptr, *rec_tstart = nullptr, *rec_off = nullptr;  // per-tile counts, starts, slice offsets
+    unsigned *rec_cstart = nullptr, *rec_cmap = nullptr;  // per-tile first chunk, chunk -> tile
+    bool defer_L = true;                 // BUGDEF_MC_DEFER_FLUENCE=0: the transport adds path lengths inline
+    int rec_per_photon = 512;            // record buffer size per photon (BUGDEF_MC_RECORDS_PER_PHOTON)
     void *staging = nullptr;             // f64 staging for conversions
     size_t staging_bytes = 0;
     void *host_pinned = nullptr;         // small pinned host block (64 doubles)
@@ -152,6 +165,7 @@ void set_error(const std::string &msg);
 int ensure(Ctx *c, void **p, size_t bytes);  // lazily allocate (and zero) a buffer
 int ensure_state(Ctx *c);                     // T, T2, H, chi, Q, k, rc, flags, ...
 int ensure_staging(Ctx *c, size_t bytes);
+int ensure_records(Ctx *c, const bd_beam *b);  // the transport's path-length record buffer (when deferred)
 
 // launch helpers implemented in the kernel translation units
 int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger);
@@ -159,10 +173,11 @@ int launch_plume_substeps(Ctx *c, int n, double dt, double decay);
 int launch_plateau(Ctx *c);
 int launch_arrhenius(Ctx *c, double A, double EaR, double dt);
 int launch_char(Ctx *c, double A, double EaR, double dt);
-int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out);
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, bool overlap_records = false);
+int join_records(Ctx *c);  // the context stream waits for record passes left on the side stream
 // enqueue only (no host wait); with ep != null the photon key comes from the device params
 // timed: 0 no events, 1 ev0/ev1 around the transport kernel, 2 graph-external gev[0]/gev[1]
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed);
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, bool overlap_records = false);
 // One transport launch for n contexts (each with power on, EpochParams in c->ep):
 // prepares each on its own stream, traces all photons in one kernel on a
 // library stream, then queues each context's absorbed-power conversion on its

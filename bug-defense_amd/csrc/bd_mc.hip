@@ -25,6 +25,7 @@
 // ceiling, bd_probe_atomic_rate).
 #include "bd_internal.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <mutex>
@@ -35,7 +36,8 @@ namespace bd {
 struct McArgs {
     const float4 *__restrict__ opt;   // {mu_a, mu_s, g, n}
     float2 *__restrict__ AL;
-    double *__restrict__ acc;         // 0 spec, 1 diffuse, 2 trans, 3 lateral, 4 absorbed, 5 events, 6 tally atomics
+    double *__restrict__ acc;         // 0 spec, 1 diffuse, 2 trans, 3 lateral, 4 absorbed, 5 events,
+                                      // 6 tally atomics, 7 path-length records
     double *__restrict__ lat;
     unsigned long long *__restrict__ counter;
     int Nx, Ny, Nz;
@@ -52,14 +54,31 @@ struct McArgs {
     unsigned k0, k1;
     int score_L;
     const EpochParams *ep;            // when set, the photon key is mix_seed(seed_base, epoch)
+    // deferred path-length tally: when rec is set, L contributions go to the record buffer as
+    // {voxel, float bits} (REC_NONE marks an unused slot), added afterwards by the record passes
+    uint2 *__restrict__ rec;
+    unsigned long long *__restrict__ rec_cursor;  // record slots handed out, in chunks of REC_CHUNK
+    long long rec_cap;
 };
 
 constexpr unsigned TAG = 0x6d63u;
+constexpr unsigned REC_NONE = 0xffffffffu;
+constexpr int REC_CHUNK = 512;
 
+// A photon's stream is the sequence of Philox blocks (blk = 0, 1, 2, ...), each giving four
+// 32-bit draws, consumed in order.  The lane holds the block being consumed (c*, from index idx)
+// and the following block (n*), generated ahead.  A draw never runs Philox itself: when the
+// current block is used up it takes the next one and marks it for regeneration (need).  The
+// only Philox is in rng_top(), called where the wave is converged (top of every event, launch,
+// each trial of the rejection sampler); with the next block present that leaves at least four
+// draws, the most any one event takes.  Inlined at every draw site, Philox (10 rounds) was
+// issued once per site for the whole wave whenever any lane there needed a block.
 struct Rng {
     unsigned k0, k1, p0, p1, blk;
-    unsigned b0, b1, b2, b3;
+    unsigned c0, c1, c2, c3;
+    unsigned n0, n1, n2, n3;
     int idx;
+    bool need;
 };
 
 __device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1,
@@ -85,14 +104,38 @@ __device__ __forceinline__ void philox(unsigned c0, unsigned c1, unsigned c2, un
     o3 = c3;
 }
 
+__device__ __forceinline__ void rng_top(Rng &r)
+{
+    if (r.need) {
+        philox(r.blk, r.p0, r.p1, TAG, r.k0, r.k1, r.n0, r.n1, r.n2, r.n3);
+        r.blk++;
+        r.need = false;
+    }
+}
+
+// A fresh stream for photon id pid: nothing consumed, block 0 still to generate (by rng_top).
+__device__ __forceinline__ void rng_start(Rng &r, unsigned k0, unsigned k1, unsigned long long pid)
+{
+    r.k0 = k0;
+    r.k1 = k1;
+    r.p0 = (unsigned)pid;
+    r.p1 = (unsigned)(pid >> 32);
+    r.blk = 0;
+    r.idx = 4;
+    r.need = true;
+}
+
 __device__ __forceinline__ float rng_u(Rng &r)
 {
     if (r.idx == 4) {
-        philox(r.blk, r.p0, r.p1, TAG, r.k0, r.k1, r.b0, r.b1, r.b2, r.b3);
-        r.blk++;
+        r.c0 = r.n0;
+        r.c1 = r.n1;
+        r.c2 = r.n2;
+        r.c3 = r.n3;
         r.idx = 0;
+        r.need = true;
     }
-    const unsigned x = r.idx == 0 ? r.b0 : (r.idx == 1 ? r.b1 : (r.idx == 2 ? r.b2 : r.b3));
+    const unsigned x = r.idx == 0 ? r.c0 : (r.idx == 1 ? r.c1 : (r.idx == 2 ? r.c2 : r.c3));
     r.idx++;
     return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);
 }
@@ -165,7 +208,21 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
     float pA = 0.f, pL = 0.f;
     double t_spec = 0.0, t_diff = 0.0, t_trans = 0.0, t_lat = 0.0, t_abs = 0.0;
     long long t_ev = 0;
-    unsigned t_at = 0;  // voxel-tally atomics this lane issued
+    unsigned t_at = 0;   // voxel-tally atomics this lane issued
+    unsigned t_rec = 0;  // path-length records this lane appended
+
+    // Deferred path-length tally (a.rec set).  A photon adds a path length to nearly every voxel
+    // it crosses, and consecutive crossings are different voxels, so those additions do not
+    // coalesce per lane; as atomics they are most of the transport's time.  Instead the lane's
+    // contribution to a voxel it leaves waits in (ov, oL) until the wave's next append, which
+    // writes every waiting record of the wave with one coalesced store into the wave's chunk of
+    // the record buffer.  rec_append() runs only where all 64 lanes are active, so the chunk
+    // bounds (wcur, wlim) and `defer` stay identical across the wave.  A lane with a record
+    // already waiting (a second voxel change in one event) adds the new one inline.
+    bool defer = a.rec != nullptr;
+    int ov = -1;
+    float oL = 0.f;
+    int wcur = 0, wlim = 0;  // record slots; the capacity is at most 2^28
 
     auto flush = [&]() {
         if (pv >= 0) {
@@ -175,8 +232,13 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                 ++t_at;
             }
             if (a.score_L && pL != 0.f) {
-                unsafeAtomicAdd(cell + 1, pL);
-                ++t_at;
+                if (defer && ov < 0) {
+                    ov = pv;
+                    oL = pL;
+                } else {
+                    unsafeAtomicAdd(cell + 1, pL);
+                    ++t_at;
+                }
             }
         }
         pv = -1;
@@ -197,6 +259,40 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
         alive = false;
         flush();
     };
+    auto rec_append = [&]() {
+        if (!defer) return;
+        const unsigned long long m = __ballot(ov >= 0);
+        if (!m) return;
+        const int cnt = __popcll(m);
+        if (wcur + cnt > wlim) {
+            // close the chunk (its unused slots become REC_NONE) and take the next one
+            for (int i = wcur + lane; i < wlim; i += 64) a.rec[i] = make_uint2(REC_NONE, 0u);
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(a.rec_cursor, (unsigned long long)REC_CHUNK);
+            base = __shfl(base, 0);
+            if ((long long)base + REC_CHUNK > a.rec_cap) {
+                // buffer full: its slots from base on become REC_NONE, and this wave adds inline
+                for (long long i = (long long)base + lane; i < a.rec_cap; i += 64) a.rec[i] = make_uint2(REC_NONE, 0u);
+                defer = false;
+                wcur = wlim = 0;
+                if (ov >= 0) {
+                    unsafeAtomicAdd((float *)&a.AL[ov] + 1, oL);
+                    ++t_at;
+                    ov = -1;
+                }
+                return;
+            }
+            wcur = __builtin_amdgcn_readfirstlane((int)base);
+            wlim = wcur + REC_CHUNK;
+        }
+        if (ov >= 0) {
+            const int rank = __popcll(m & ((1ull << lane) - 1ull));
+            a.rec[wcur + rank] = make_uint2((unsigned)ov, __float_as_uint(oL));
+            ov = -1;
+            ++t_rec;
+        }
+        wcur = __builtin_amdgcn_readfirstlane(wcur + cnt);
+    };
 
     for (;;) {
         // ---- refill idle lanes from the global photon counter
@@ -216,12 +312,8 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                 } else {
                     // ---- launch (voxel_mc_3d.py:150-194)
                     const unsigned long long pid = (unsigned long long)a.photon_offset + my;
-                    rng.k0 = key0;
-                    rng.k1 = key1;
-                    rng.p0 = (unsigned)pid;
-                    rng.p1 = (unsigned)(pid >> 32);
-                    rng.blk = 0;
-                    rng.idx = 4;
+                    rng_start(rng, key0, key1, pid);
+                    rng_top(rng);  // four draws: at most three are taken below outside the sampler loop
                     float u1, u2;
                     if (a.profile == 1) {
                         const float rr = a.R * sqrtf(rng_u(rng));
@@ -231,6 +323,7 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                         u2 = rr * s;
                     } else if (a.profile == 2 || a.profile == 3) {
                         for (;;) {
+                            rng_top(rng);  // a trial takes at most three draws, leaving one for tau
                             const float rn = sqrtf(rng_u(rng));
                             const float accp = a.profile == 2 ? __expf(-2.f * rn) : __expf(-2.f * rn * rn);
                             if (rng_u(rng) < accp) {
@@ -301,8 +394,13 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
             continue;
         }
         // ---- transport a burst of events, then refill
+        // (a lane whose packet ended idles to the end of the burst, so that every lane is
+        // active at rec_append)
         for (int it = 0; it < 16; ++it) {
-            if (!alive) break;
+            rec_append();
+            if (!__any(alive)) break;
+            if (!alive) continue;
+            rng_top(rng);  // an event takes at most four draws (roulette, polar, azimuth, tau)
             const int v = (p.ix * Ny + p.iy) * Nz + p.iz;
             const float mua = p.o.x;
             const float mut = p.o.x + p.o.y;
@@ -481,12 +579,16 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                 flush();
             }
         }
+        rec_append();
     }
     flush();
+    rec_append();
+    if (defer)
+        for (int i = wcur + lane; i < wlim; i += 64) a.rec[i] = make_uint2(REC_NONE, 0u);
     // ---- wave reduction of the FP64 scalar tallies, one atomic per wave
-    double vals[7] = {t_spec, t_diff, t_trans, t_lat, t_abs, (double)t_ev, (double)t_at};
+    double vals[8] = {t_spec, t_diff, t_trans, t_lat, t_abs, (double)t_ev, (double)t_at, (double)t_rec};
 #pragma unroll
-    for (int q = 0; q < 7; ++q) {
+    for (int q = 0; q < 8; ++q) {
         double v = vals[q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
         if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.acc[q], v);
@@ -550,6 +652,146 @@ int launch_pack_optics(Ctx *c, bool add_plume)
 }
 
 // Q = A * power / (N * V)  (voxel_mc_3d.py:684)
+// ---- the deferred path-length records of a transport launch, added into the L tally: a
+// counting sort of the records by 1024-voxel tile (a tile is a range of the linear voxel index),
+// then chunks of at most REC_CHUNK_RECS records of one tile are summed in LDS and the sums added
+// to L, with plain stores where a tile is one chunk and with atomics (at most one per voxel and
+// chunk, far fewer than records) where several chunks share a tile.  The slots below
+// min(cursor, capacity) hold records or REC_NONE (the transport closes its chunks).
+constexpr int REC_TILE_BITS = 10;
+constexpr int REC_TILE = 1 << REC_TILE_BITS;   // voxels per tile: 4 KB of float sums in LDS
+constexpr int REC_SLICES = 512;                // record slices of the count and scatter passes
+constexpr int REC_MAX_TILES = 16384;           // tile counters in LDS (64 KB): grids up to 2^24 voxels
+constexpr int REC_CHUNK_RECS = 32768;          // records per accumulation chunk
+
+__device__ __forceinline__ void rec_slice(const unsigned long long *cursor, long long cap, long long &lo,
+                                          long long &hi)
+{
+    const long long n = (long long)*cursor < cap ? (long long)*cursor : cap;
+    const long long per = (n + REC_SLICES - 1) / REC_SLICES;
+    lo = (long long)blockIdx.x * per;
+    hi = lo + per < n ? lo + per : n;
+}
+
+// Pass 1 (REC_SLICES blocks): the slice's records per tile; one returning atomic per touched
+// tile reserves the slice's range inside the tile (off) and counts the tile (tcnt).
+__global__ __launch_bounds__(256) void k_rec_count(const uint2 *__restrict__ rec,
+                                                   const unsigned long long *__restrict__ cursor, long long cap,
+                                                   unsigned nvox, int ntiles, unsigned *__restrict__ tcnt,
+                                                   unsigned *__restrict__ off)
+{
+    extern __shared__ unsigned h[];
+    for (int t = threadIdx.x; t < ntiles; t += 256) h[t] = 0u;
+    __syncthreads();
+    long long lo, hi;
+    rec_slice(cursor, cap, lo, hi);
+    for (long long i = lo + threadIdx.x; i < hi; i += 256) {
+        const unsigned v = rec[i].x;
+        if (v < nvox) atomicAdd(&h[v >> REC_TILE_BITS], 1u);
+    }
+    __syncthreads();
+    unsigned *o = off + (size_t)blockIdx.x * ntiles;
+    for (int t = threadIdx.x; t < ntiles; t += 256)
+        if (h[t]) o[t] = atomicAdd(&tcnt[t], h[t]);
+}
+
+// Pass 2 (one block of 1024): exclusive scans of the tile counts (tstart, ntiles + 1 entries)
+// and of the tiles' chunk counts (cstart), and the chunk -> tile map (cmap); the counts are
+// left zeroed for the next launch.
+__global__ __launch_bounds__(1024) void k_rec_scan(unsigned *__restrict__ tcnt, int ntiles,
+                                                   unsigned *__restrict__ tstart, unsigned *__restrict__ cstart,
+                                                   unsigned *__restrict__ cmap)
+{
+    __shared__ unsigned part[1024], cpart[1024];
+    const int per = (ntiles + 1023) / 1024;
+    const int lo = threadIdx.x * per, hi = lo + per < ntiles ? lo + per : ntiles;
+    unsigned s = 0, cs = 0;
+    for (int t = lo; t < hi; ++t) {
+        s += tcnt[t];
+        cs += (tcnt[t] + REC_CHUNK_RECS - 1) / REC_CHUNK_RECS;
+    }
+    part[threadIdx.x] = s;
+    cpart[threadIdx.x] = cs;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scans of the thread sums
+        const unsigned x = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        const unsigned y = threadIdx.x >= d ? cpart[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        cpart[threadIdx.x] += y;
+        __syncthreads();
+    }
+    unsigned run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    unsigned crun = threadIdx.x ? cpart[threadIdx.x - 1] : 0u;
+    for (int t = lo; t < hi; ++t) {
+        const unsigned n = tcnt[t];
+        tcnt[t] = 0u;  // cleared for the next launch (this pass is the counts' last reader)
+        tstart[t] = run;
+        cstart[t] = crun;
+        const unsigned nch = (n + REC_CHUNK_RECS - 1) / REC_CHUNK_RECS;
+        for (unsigned k = 0; k < nch; ++k) cmap[crun + k] = (unsigned)t;
+        run += n;
+        crun += nch;
+    }
+    if (threadIdx.x == 1023) {
+        tstart[ntiles] = part[1023];
+        cstart[ntiles] = cpart[1023];
+    }
+}
+
+// Pass 3 (REC_SLICES blocks, the slices of pass 1): each record to its place in tile order.
+__global__ __launch_bounds__(256) void k_rec_scatter(const uint2 *__restrict__ rec,
+                                                     const unsigned long long *__restrict__ cursor, long long cap,
+                                                     unsigned nvox, int ntiles, const unsigned *__restrict__ off,
+                                                     const unsigned *__restrict__ tstart, uint2 *__restrict__ sorted)
+{
+    extern __shared__ unsigned cur[];
+    const unsigned *o = off + (size_t)blockIdx.x * ntiles;
+    long long lo, hi;
+    rec_slice(cursor, cap, lo, hi);
+    // only tiles this slice touched have an off entry; the others are never indexed below
+    for (int t = threadIdx.x; t < ntiles; t += 256) cur[t] = tstart[t] + o[t];
+    __syncthreads();
+    for (long long i = lo + threadIdx.x; i < hi; i += 256) {
+        const uint2 r = rec[i];
+        if (r.x < nvox) sorted[atomicAdd(&cur[r.x >> REC_TILE_BITS], 1u)] = r;
+    }
+}
+
+// Pass 4 (a fixed grid over the chunks): a chunk's records summed in LDS, the sums added to L.
+__global__ __launch_bounds__(512) void k_rec_accum(const uint2 *__restrict__ sorted,
+                                                   const unsigned *__restrict__ tstart,
+                                                   const unsigned *__restrict__ cstart,
+                                                   const unsigned *__restrict__ cmap, int ntiles, unsigned nvox,
+                                                   float2 *__restrict__ AL)
+{
+    __shared__ float acc[REC_TILE];
+    const unsigned nchunks = cstart[ntiles];
+    for (unsigned j = blockIdx.x; j < nchunks; j += gridDim.x) {
+        const unsigned t = cmap[j];
+        const unsigned k = j - cstart[t];
+        const unsigned s = tstart[t] + k * REC_CHUNK_RECS;
+        const unsigned e = min(tstart[t + 1], s + REC_CHUNK_RECS);
+        const bool shared_tile = cstart[t + 1] - cstart[t] > 1;
+        for (int q = threadIdx.x; q < REC_TILE; q += 512) acc[q] = 0.f;
+        __syncthreads();
+        for (unsigned i = s + threadIdx.x; i < e; i += 512) {
+            const uint2 r = sorted[i];
+            atomicAdd(&acc[r.x & (REC_TILE - 1)], __uint_as_float(r.y));
+        }
+        __syncthreads();
+        const unsigned base = t << REC_TILE_BITS;
+        for (int q = threadIdx.x; q < REC_TILE; q += 512) {
+            const float x = acc[q];
+            if (x != 0.f && base + q < nvox) {
+                if (shared_tile) unsafeAtomicAdd((float *)&AL[base + q] + 1, x);
+                else AL[base + q].y += x;
+            }
+        }
+        __syncthreads();  // acc is cleared for the next chunk only after every thread has read it
+    }
+}
+
 __global__ void k_q_from_a(const float2 *__restrict__ AL, float *__restrict__ Q, float scale, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -603,10 +845,20 @@ static void fill_mc_args(const bd_grid &g, const bd_beam *b, McArgs &a)
     a.score_L = b->score_fluence ? 1 : 0;
 }
 
+int join_records(Ctx *c)
+{
+    if (c->join_pending) {
+        BD_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        c->join_pending = false;
+    }
+    return BD_OK;
+}
+
 // Pack the optics, clear the tallies and fill the kernel arguments of one
 // context's transport (no launch).
 static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a)
 {
+    BD_TRY(join_records(c));  // the tallies and record buffers are about to be reused
     BD_TRY(launch_pack_optics(c, b->add_plume != 0));
     BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
     BD_TRY(ensure(c, (void **)&c->Q, (size_t)c->nvox * sizeof(float)));
@@ -632,14 +884,27 @@ static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a
     return BD_OK;
 }
 
-// Blocks for n photons: one photon per lane, at most 8 blocks per CU (idle lanes
-// refill from the photon counter when there are more photons than lanes).
+// Blocks of k_mc that are resident at once on one CU (its register use sets it), at most 8.
+static int mc_resident_per_cu()
+{
+    static int n = 0;
+    if (n == 0) {
+        int r = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, (const void *)k_mc, 256, 0) != hipSuccess || r < 1) r = 8;
+        n = r < 8 ? r : 8;
+    }
+    return n;
+}
+
+// Blocks for n photons: one photon per lane, at most as many blocks as are resident at once
+// (idle lanes refill from the photon counter, so photons beyond the resident lanes go to the
+// lanes whose photons end first instead of to blocks that wait for a free slot).
 static int64_t mc_blocks(Ctx *c, long long n_photons)
 {
     int ncu = 256;
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     const int64_t want = (n_photons + 255) / 256;
-    int64_t cap = (int64_t)ncu * 8;
+    int64_t cap = (int64_t)ncu * mc_resident_per_cu();
     if (c->mc_blocks_per_cu > 0) cap = (int64_t)ncu * c->mc_blocks_per_cu;
     return want < cap ? (want < 1 ? 1 : want) : cap;
 }
@@ -654,10 +919,76 @@ static int mc_post(Ctx *c, const bd_beam *b)
     return BD_OK;
 }
 
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed)
+// Record buffer of the deferred path-length tally: rec_per_photon records per photon (512 by
+// default; a photon of the layered-stack workloads leaves 150-400), between 2^16 and 2^28
+// records (0.5 MB - 2 GB).  A launch that fills it adds the rest inline, so the size bounds only
+// how much is deferred.
+int ensure_records(Ctx *c, const bd_beam *b)
+{
+    const int64_t ntiles = (c->nvox + REC_TILE - 1) >> REC_TILE_BITS;
+    if (!c->defer_L || !b->score_fluence || ntiles > REC_MAX_TILES) return BD_OK;
+    int64_t cap = (int64_t)b->n_photons * c->rec_per_photon;
+    cap = std::min<int64_t>(std::max<int64_t>(cap, int64_t(1) << 16), int64_t(1) << 28);
+    if (!c->rec_tcnt) {
+        BD_HIP(hipMalloc((void **)&c->rec_tcnt, sizeof(unsigned) * (size_t)ntiles));
+        BD_HIP(hipMemset(c->rec_tcnt, 0, sizeof(unsigned) * (size_t)ntiles));  // k_rec_scan re-zeroes them
+        BD_HIP(hipMalloc((void **)&c->rec_tstart, sizeof(unsigned) * (size_t)(ntiles + 1)));
+        BD_HIP(hipMalloc((void **)&c->rec_cstart, sizeof(unsigned) * (size_t)(ntiles + 1)));
+        BD_HIP(hipMalloc((void **)&c->rec_off, sizeof(unsigned) * (size_t)REC_SLICES * ntiles));
+    }
+    if (c->rec_cap >= cap) return BD_OK;
+    if (c->rec) BD_HIP(hipFree(c->rec));
+    if (c->rec_sorted) BD_HIP(hipFree(c->rec_sorted));
+    if (c->rec_cmap) BD_HIP(hipFree(c->rec_cmap));
+    c->rec = c->rec_sorted = nullptr;
+    c->rec_cmap = nullptr;
+    c->rec_cap = 0;
+    BD_HIP(hipMalloc((void **)&c->rec, sizeof(uint2) * (size_t)cap));
+    BD_HIP(hipMalloc((void **)&c->rec_sorted, sizeof(uint2) * (size_t)cap));
+    // chunks: at most one partial chunk per tile beyond cap / REC_CHUNK_RECS full ones
+    BD_HIP(hipMalloc((void **)&c->rec_cmap, sizeof(unsigned) * (size_t)(cap / REC_CHUNK_RECS + ntiles + 1)));
+    c->rec_cap = cap;
+    return BD_OK;
+}
+
+// The record passes after a deferred transport launch, on stream st.
+static int launch_record_passes(Ctx *c, hipStream_t st)
+{
+    const int ntiles = (int)((c->nvox + REC_TILE - 1) >> REC_TILE_BITS);
+    const size_t lds = sizeof(unsigned) * (size_t)ntiles;
+    const unsigned nv = (unsigned)c->nvox;
+    hipLaunchKernelGGL(k_rec_count, dim3(REC_SLICES), dim3(256), lds, st, c->rec, c->counter + 3,
+                       (long long)c->rec_cap, nv, ntiles, c->rec_tcnt, c->rec_off);
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(1024), 0, st, c->rec_tcnt, ntiles, c->rec_tstart,
+                       c->rec_cstart, c->rec_cmap);
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rec_scatter, dim3(REC_SLICES), dim3(256), lds, st, c->rec, c->counter + 3,
+                       (long long)c->rec_cap, nv, ntiles, c->rec_off, c->rec_tstart, c->rec_sorted);
+    BD_HIP(hipGetLastError());
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    hipLaunchKernelGGL(k_rec_accum, dim3(ncu * 4), dim3(512), 0, st, c->rec_sorted, c->rec_tstart,
+                       c->rec_cstart, c->rec_cmap, ntiles, nv, c->AL);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
+// The transport launch.  With deferred path lengths, the record passes follow on the context
+// stream, or, with overlap_records, on the side stream, which the caller joins (join_records)
+// before anything reads the path-length tally or the next launch reuses the record buffers.
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, bool overlap_records)
 {
     McArgs a;
     BD_TRY(mc_prepare(c, b, ep, a));
+    BD_TRY(ensure_records(c, b));
+    const bool defer = c->defer_L && b->score_fluence && c->rec != nullptr;
+    if (defer) {
+        BD_TRY(launch_fill(c, c->counter + 3, K_F64, 1, 0.0));
+        a.rec = c->rec;
+        a.rec_cursor = c->counter + 3;
+        a.rec_cap = c->rec_cap;
+    }
     const int nblk = (int)mc_blocks(c, b->n_photons);
     if (timed == 1) BD_HIP(hipEventRecord(c->ev0, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[0], c->stream, hipEventRecordExternal));
@@ -665,6 +996,20 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
     BD_HIP(hipGetLastError());
     if (timed == 1) BD_HIP(hipEventRecord(c->ev1, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[1], c->stream, hipEventRecordExternal));
+    hipStream_t rs = c->stream;
+    if (defer && overlap_records) {
+        BD_HIP(hipEventRecord(c->ev_fork, c->stream));
+        BD_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        rs = c->side;
+    }
+    if (timed == 1) BD_HIP(hipEventRecord(c->ev7, rs));
+    if (defer) BD_TRY(launch_record_passes(c, rs));
+    if (timed == 1) BD_HIP(hipEventRecord(c->ev6, rs));
+    if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[6], rs, hipEventRecordExternal));
+    if (rs != c->stream) {
+        BD_HIP(hipEventRecord(c->ev_join, rs));
+        c->join_pending = true;
+    }
     return mc_post(c, b);
 }
 
@@ -750,7 +1095,10 @@ int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n)
     for (int i = 0; i < n; ++i) BD_HIP(hipEventRecord(cs[i]->gev[0], g->stream));
     hipLaunchKernelGGL(k_mc_gang, dim3((unsigned)m), dim3(256), 0, g->stream, g->d_args, g->d_map);
     BD_HIP(hipGetLastError());
-    for (int i = 0; i < n; ++i) BD_HIP(hipEventRecord(cs[i]->gev[1], g->stream));
+    for (int i = 0; i < n; ++i) {
+        BD_HIP(hipEventRecord(cs[i]->gev[1], g->stream));
+        BD_HIP(hipEventRecord(cs[i]->gev[6], g->stream));  // no record pass: the gang transport adds inline
+    }
     BD_HIP(hipEventRecord(g->done, g->stream));
     for (int i = 0; i < n; ++i) {
         BD_HIP(hipStreamWaitEvent(cs[i]->stream, g->done, 0));
@@ -759,10 +1107,10 @@ int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n)
     return BD_OK;
 }
 
-int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, bool overlap_records)
 {
-    BD_TRY(launch_mc_enqueue(c, b, nullptr, 1));
-    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_TRY(launch_mc_enqueue(c, b, nullptr, 1, overlap_records));
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;
     out->specular = r[0];
@@ -772,9 +1120,16 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
     out->absorbed = r[4];
     out->events = r[5];
     out->tally_atomics = r[6];
+    out->tally_records = r[7];
     float ms = 0.f;
     BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     out->kernel_ms = ms;  // the transport kernel alone
+    if (c->join_pending) {
+        out->record_ms = std::nan("");  // still running on the side stream; the caller times it after joining
+    } else {
+        BD_HIP(hipEventElapsedTime(&ms, c->ev7, c->ev6));
+        out->record_ms = ms;  // the path-length record passes (0 when the transport adds them inline)
+    }
     return BD_OK;
 }
 
@@ -884,6 +1239,8 @@ extern "C" int bd_mc_run_noalloc(const bd_grid *grid, const bd_beam *beam, const
     out->absorbed = acc[4];
     out->events = acc[5];
     out->tally_atomics = acc[6];
+    out->tally_records = 0.0;  // this entry point adds path lengths inline (no record buffer)
+    out->record_ms = 0.0;
     out->kernel_ms = -1.0;  // not measured: timing events would be allocations
     return BD_OK;
 }

@@ -80,7 +80,7 @@ class Beam(ctypes.Structure):
 class MCResult(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("specular", "diffuse", "transmitted", "lateral", "absorbed", "events", "kernel_ms",
-                 "tally_atomics")]
+                 "tally_atomics", "tally_records", "record_ms")]
 
 
 class EpochDiag(ctypes.Structure):
@@ -88,7 +88,8 @@ class EpochDiag(ctypes.Structure):
         "absorbed_frac", "mc_energy_closure", "plume_tau", "plume_transmission", "plume_peak_mu_a",
         "plume_mean_mu_a", "t_optical_ms", "t_thermal_ms", "t_phase_ms", "absorbed_J", "reflected_J",
         "transmitted_J", "lateral_J", "conducted_J", "ablated_J", "R_spec", "R_diff", "transmission",
-        "lateral_loss", "mc_events", "t_substeps_ms", "t_mc_kernel_ms", "mc_tally_atomics")]
+        "lateral_loss", "mc_events", "t_substeps_ms", "t_mc_kernel_ms", "mc_tally_atomics",
+        "mc_tally_records", "t_mc_records_ms")]
 
 
 _lib = None

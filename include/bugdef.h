@@ -120,7 +120,9 @@ typedef struct {
     double specular, diffuse, transmitted, lateral, absorbed; /* weight sums */
     double events;                 /* transport events traced */
     double kernel_ms;              /* device time of the transport kernel */
-    double tally_atomics;          /* float atomic adds issued into the A/L voxel tallies */
+    double tally_atomics;          /* float atomic adds the transport kernel issued into the A/L voxel tallies */
+    double tally_records;          /* path-length additions the kernel deferred to the record pass */
+    double record_ms;              /* device time of the record pass that adds them (0: none deferred) */
 } bd_mc_result;
 
 /* Per-epoch diagnostics (coupled_ablation.py:605-617) plus FP64 ledgers. */
@@ -134,7 +136,9 @@ typedef struct {
     double mc_events;
     double t_substeps_ms;          /* device time of the fused thermal sub-step launches alone */
     double t_mc_kernel_ms;         /* device time of the transport kernel alone */
-    double mc_tally_atomics;       /* float atomic adds the transport issued into the A/L tallies */
+    double mc_tally_atomics;       /* float atomic adds the transport kernel issued into the A/L tallies */
+    double mc_tally_records;       /* path-length additions deferred to the record pass */
+    double t_mc_records_ms;        /* device time of the record pass */
 } bd_epoch_diag;
 
 /* Device / library */

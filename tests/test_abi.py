@@ -33,8 +33,8 @@ def test_struct_sizes_match_header():
     """The ctypes mirrors must have the C layout (a mismatch corrupts every call)."""
     from simulation import _bdlib as B
     assert ctypes.sizeof(B.Grid) == 3 * 4 + 4 + 3 * 8  # int32 x3, pad, double x3
-    assert ctypes.sizeof(B.MCResult) == 8 * 8
-    assert ctypes.sizeof(B.EpochDiag) == 8 + 23 * 8
+    assert ctypes.sizeof(B.MCResult) == 10 * 8
+    assert ctypes.sizeof(B.EpochDiag) == 8 + 25 * 8
     assert ctypes.sizeof(B.FnoSpec) == 4 * 4
 
 

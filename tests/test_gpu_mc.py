@@ -163,7 +163,9 @@ def test_tally_atomic_count_and_probe():
                              score_fluence=bool(sf))
             res[sf] = ctx.mc_run(beam)
         n_a = res[0].tally_atomics
-        n_l = res[1].tally_atomics - n_a
+        # path-length additions: atomics the kernel issued plus those it deferred as records
+        n_l = res[1].tally_atomics + res[1].tally_records - n_a
+        assert res[0].tally_records == 0
         assert res[0].events == res[1].events
         assert n_a > 0 and n_l >= n_a
         assert n_l <= res[1].events + 20_000
@@ -171,6 +173,75 @@ def test_tally_atomic_count_and_probe():
         ctx.close()
     rate = B.probe_atomic_rate(int(np.prod(shape)), n_pairs=4096 * 64)
     assert rate > 1e8
+
+
+def _mc_context(shape, d, props, defer, records_per_photon=None):
+    """A context whose transport defers the path-length tally (or adds it inline), and its
+    record buffer size, set by the BUGDEF_MC_DEFER_FLUENCE / BUGDEF_MC_RECORDS_PER_PHOTON knobs
+    that bd_create reads."""
+    import os
+    from simulation import _bdlib as B
+    knobs = {"BUGDEF_MC_DEFER_FLUENCE": "1" if defer else "0"}
+    if records_per_photon is not None:
+        knobs["BUGDEF_MC_RECORDS_PER_PHOTON"] = str(records_per_photon)
+    old = {k: os.environ.get(k) for k in knobs}
+    os.environ.update(knobs)
+    try:
+        ctx = B.Context(shape, d)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    for name, key in (('MU_A', 'mu_a'), ('MU_S', 'mu_s'), ('G', 'g'), ('N', 'n')):
+        ctx.upload(name, np.ascontiguousarray(props[key], dtype=np.float32))
+    return ctx
+
+
+@pytest.mark.parametrize("case", ["layered", "record_buffer_full"])
+def test_deferred_path_length_tally_matches_inline(case):
+    """The same photons with the path-length tally deferred to records (the default) and added
+    inline by the transport: equal event counts and scalar tallies, and A and L grids equal up
+    to the order of the float additions.  'record_buffer_full': a buffer of 4 records per photon
+    (the 2^16 minimum) fills, so part of the tally goes inline after the records (both halves
+    must add up)."""
+    from simulation.voxel_mc_3d import make_beam
+    shape, d, props = (30, 30, 40), (2.0 / 30, 2.0 / 30, 1.0 / 40), _three_layer()
+    beam = make_beam(1.0, 25_000, 0.5, 22.0, 0.0, (1.0, 1.0, 0.0), 0.0, 0, 'gaussian', 77)
+    per_photon = 4 if case == "record_buffer_full" else None
+    out = {}
+    for defer in (True, False):
+        ctx = _mc_context(shape, d, props, defer, per_photon)
+        try:
+            r = ctx.mc_run(beam)
+            out[defer] = (r, ctx.download('A'), ctx.download('L'))
+            if defer:
+                beam.score_fluence = 0
+                n_a = ctx.mc_run(beam).tally_atomics  # absorbed-weight atomics alone
+                beam.score_fluence = 1
+        finally:
+            ctx.close()
+    (rd, Ad, Ld), (ri, Ai, Li) = out[True], out[False]
+    assert rd.events == ri.events
+    assert ri.tally_records == 0 and rd.tally_records > 0
+    for key in ('specular', 'diffuse', 'transmitted', 'lateral', 'absorbed'):
+        assert getattr(rd, key) == pytest.approx(getattr(ri, key), rel=1e-12, abs=1e-9), key
+    # every path-length addition is made once: inline atomics + records against the inline run
+    assert rd.tally_atomics + rd.tally_records == ri.tally_atomics
+    for g_d, g_i in ((Ad, Ai), (Ld, Li)):
+        assert g_d.sum() == pytest.approx(g_i.sum(), rel=1e-5)
+        assert np.all(np.abs(g_d - g_i) <= 2e-4 * np.abs(g_i) + 1e-7 * g_i.max())
+    n_l = ri.tally_atomics - n_a  # path-length additions of the launch
+    if case == "layered":
+        # inline only when a lane already has a record waiting: the voxel change and the end of a
+        # packet in one event, at most once per photon
+        assert n_l - 25_000 <= rd.tally_records < n_l
+    else:
+        cap = max(25_000 * 4, 1 << 16)
+        assert n_l > 4 * cap
+        assert 0.5 * cap < rd.tally_records <= cap
+        assert rd.tally_atomics - n_a > 0.5 * n_l  # the rest went inline
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (1, 1, 6), (3, 2, 1), (2, 5, 3)])

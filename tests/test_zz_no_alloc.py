@@ -56,7 +56,10 @@ def _compare(shape, spacing, n_photons, seed, theta):
     al = d_al.download().reshape(shape + (2,))
     d_opt.free()
     d_al.free()
-    assert res.events == r_ctx.events and res.tally_atomics == r_ctx.tally_atomics
+    # the allocation-free transport adds every path length inline; bd_mc_run defers most of them
+    # to its record pass, so the tally additions match in total
+    assert res.events == r_ctx.events and res.tally_records == 0.0
+    assert res.tally_atomics == r_ctx.tally_atomics + r_ctx.tally_records
     assert res.events > 10 * n_photons
     for key in ('specular', 'diffuse', 'transmitted', 'lateral', 'absorbed'):
         assert getattr(res, key) == pytest.approx(getattr(r_ctx, key), rel=1e-12, abs=1e-12), key
