@@ -181,7 +181,7 @@ int bd_destroy(bd_ctx *h)
                     c->P, c->P2, c->Pext, c->pvoid, c->to_remove, c->red, c->lat, c->counter, c->staging,
                     c->splat_tmp1, c->splat_tmp2, c->kb[0], c->kb[1], c->kb[2], c->kl[0], c->kl[1], c->kl[2],
                     c->Qs, c->Qsur, c->Qaux, c->Hb, c->flagsb, c->rec, c->rec_sorted, c->rec_tcnt,
-                    c->rec_tstart, c->rec_off, c->rec_cstart, c->rec_cmap};
+                    c->rec_tstart, c->rec_off, c->rec_cstart, c->rec_cmap, c->rec_b, c->Lb[0], c->Lb[1]};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &f : c->ds)
@@ -358,7 +358,12 @@ int bd_upload(bd_ctx *h, int field, int dtype, const void *host, int64_t n)
         break;
     }
     case KAL: {
-        BD_TRY(join_records(c));
+        if (r.which == 1) {
+            BD_TRY(join_records(c));
+            lag_drop(c);  // the uploaded path lengths replace those of a lagged launch
+        } else {
+            BD_TRY(lag_finalize(c));
+        }
         BD_TRY(ensure(c, (void **)&c->AL, (size_t)n * sizeof(float2)));
         BD_TRY(ensure_staging(c, (size_t)n * 12));
         float *tmp = (float *)((char *)c->staging + (size_t)n * 8);
@@ -421,7 +426,7 @@ int bd_download(bd_ctx *h, int field, int dtype, void *host, int64_t n)
         BD_HIP(hipMemcpyAsync(host, c->staging, (size_t)n, hipMemcpyDeviceToHost, c->stream));
         break;
     case KAL: {
-        BD_TRY(join_records(c));
+        BD_TRY(lag_finalize(c));
         BD_TRY(ensure_staging(c, (size_t)n * 12));
         float *tmp = (float *)((char *)c->staging + (size_t)n * 8);
         al_get(c, tmp, r.which);
@@ -491,6 +496,7 @@ int bd_download_slice(bd_ctx *h, int field, int plane, int index, int stride, fl
         set_error("bd_download_slice: field has never been written");
         return BD_ESTATE;
     }
+    if (r.kind == KAL) BD_TRY(lag_finalize(c));
     BD_TRY(ensure_staging(c, (size_t)n * 4));
     if (plume_field) {
         // plume cells under the thermal plane's cell centres (identity for equal grids)
@@ -713,7 +719,7 @@ int bd_reduce(bd_ctx *h, int field, int op, int mask_field, double *out)
         bits = m.kind == KFLAG ? m.bit : 0xFF;
     }
     if (r.kind == KAL) {
-        BD_TRY(join_records(c));
+        BD_TRY(lag_finalize(c));
         BD_TRY(ensure_staging(c, (size_t)c->nvox * 4));
         al_get(c, (float *)c->staging, r.which);
         return launch_reduce(c, c->staging, K_F32, r.n, op, mask, bits, out);
@@ -744,6 +750,7 @@ int bd_phase_map(bd_ctx *h, const uint8_t *is_air_gap, uint8_t *out, int64_t n)
 int bd_sync(bd_ctx *h)
 {
     BD_CHECK(h, "bd_sync: null argument");
+    BD_TRY(lag_finalize(&h->c));  // every launched transport's path-length tally is complete
     BD_HIP(hipStreamSynchronize(h->c.stream));
     return BD_OK;
 }

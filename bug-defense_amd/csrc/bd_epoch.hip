@@ -68,7 +68,7 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         out->mc_energy_closure = std::nan("");
     } else if (beam->power > 0.0) {
         bd_mc_result mc{};
-        BD_TRY(launch_mc(c, beam, &mc, true));  // record passes beside the sub-steps, joined below
+        BD_TRY(launch_mc(c, beam, &mc, REC_LAG));  // this epoch's record passes run at the next one
         const double N = (double)beam->n_photons;
         out->absorbed_frac = mc.absorbed / N;
         out->mc_energy_closure = (mc.specular + mc.diffuse + mc.transmitted + mc.lateral + mc.absorbed) / N;
@@ -83,12 +83,13 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         out->mc_tally_atomics = mc.tally_atomics;
         out->mc_tally_records = mc.tally_records;
         out->t_mc_kernel_ms = mc.kernel_ms;
-        out->t_mc_records_ms = mc.record_ms;  // NaN while they run beside the sub-steps
+        out->t_mc_records_ms = mc.record_ms;  // NaN while the previous epoch's run beside the transport
         double qsum = 0.0;
         BD_TRY(reduce_masked(c, c->Q, c->nvox, c->flags, F_ABL, true, &qsum));
         out->absorbed_J = qsum * V * dt_epoch;
     } else {
         BD_TRY(join_records(c));
+        lag_drop(c);  // the cleared tally supersedes a lagged launch's
         BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
         if (c->AL) BD_TRY(launch_fill(c, c->AL, K_F32, 2 * c->nvox, 0.0));
         out->absorbed_frac = 0.0;
@@ -221,7 +222,7 @@ static int enqueue_epoch_tail(Ctx *c, int n_substeps)
 
 static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
 {
-    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2, true));
+    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2, REC_SIDE));
     BD_TRY(enqueue_after_mc(c, beam));
     return enqueue_epoch_tail(c, n_substeps);
 }
@@ -241,6 +242,9 @@ namespace bd {
 // checks and buffers shared by the graph and gang epochs (no host round trip inside)
 static int prepare_enqueued_epoch(Ctx *c, int n_substeps, const char *who)
 {
+    // the enqueued epoch's transport or cleared tally supersedes the L of a lagged launch
+    BD_TRY(join_records(c));
+    lag_drop(c);
     if (!c->has_phys) {
         set_error(std::string(who) + ": bd_set_physics has not been called");
         return BD_ESTATE;

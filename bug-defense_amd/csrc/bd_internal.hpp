@@ -100,6 +100,14 @@ struct Ctx {
     int64_t rec_cap = 0;                 // records each of the two buffers holds
     unsigned *rec_tcnt = nullptr, *rec_tstart = nullptr, *rec_off = nullptr;  // per-tile counts, starts, slice offsets
     unsigned *rec_cstart = nullptr, *rec_cmap = nullptr;  // per-tile first chunk, chunk -> tile
+    // one-epoch lag of the record passes in eager epochs: the records of an epoch's transport
+    // wait in record buffer `lag` (0: rec, cursor counter[3]; 1: rec_b, counter[4]), their inline
+    // path lengths in Lb[lag], until the next eager epoch runs their passes beside its own
+    // transport (the L they give is superseded by that epoch's), or a read of the path-length
+    // tally runs them and copies Lb[lag] into AL (lag_finalize)
+    uint2 *rec_b = nullptr;
+    float *Lb[2] = {nullptr, nullptr};
+    int lag = -1;
     bool defer_L = true;                 // BUGDEF_MC_DEFER_FLUENCE=0: the transport adds path lengths inline
     int rec_per_photon = 512;            // record buffer size per photon (BUGDEF_MC_RECORDS_PER_PHOTON)
     void *staging = nullptr;             // f64 staging for conversions
@@ -173,11 +181,18 @@ int launch_plume_substeps(Ctx *c, int n, double dt, double decay);
 int launch_plateau(Ctx *c);
 int launch_arrhenius(Ctx *c, double A, double EaR, double dt);
 int launch_char(Ctx *c, double A, double EaR, double dt);
-int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, bool overlap_records = false);
-int join_records(Ctx *c);  // the context stream waits for record passes left on the side stream
+// where the record passes of a transport launch run: after it on the context stream, on the
+// side stream beside the rest of the epoch (joined at its end), or at the next eager epoch
+enum RecMode { REC_MAIN = 0, REC_SIDE = 1, REC_LAG = 2 };
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, int rec_mode = REC_MAIN);
+int join_records(Ctx *c);   // the context stream waits for record passes left on the side stream
+int lag_finalize(Ctx *c);   // the lagged record passes (if any) run and their L lands in AL
+void al_get(Ctx *c, float *d, int which);        // one half of AL (0: A, 1: L) into a float grid
+void al_set(Ctx *c, const float *s, int which);  // a float grid into one half of AL
+inline void lag_drop(Ctx *c) { c->lag = -1; }  // a new transport or a cleared tally supersedes them
 // enqueue only (no host wait); with ep != null the photon key comes from the device params
 // timed: 0 no events, 1 ev0/ev1 around the transport kernel, 2 graph-external gev[0]/gev[1]
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, bool overlap_records = false);
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, int rec_mode = REC_MAIN);
 // One transport launch for n contexts (each with power on, EpochParams in c->ep):
 // prepares each on its own stream, traces all photons in one kernel on a
 // library stream, then queues each context's absorbed-power conversion on its

@@ -59,6 +59,9 @@ struct McArgs {
     uint2 *__restrict__ rec;
     unsigned long long *__restrict__ rec_cursor;  // record slots handed out, in chunks of REC_CHUNK
     long long rec_cap;
+    // where path lengths added inline go: Lt[voxel * Ls] (the L half of AL, or a separate grid)
+    float *__restrict__ Lt;
+    int Ls;
 };
 
 constexpr unsigned TAG = 0x6d63u;
@@ -236,7 +239,7 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                     ov = pv;
                     oL = pL;
                 } else {
-                    unsafeAtomicAdd(cell + 1, pL);
+                    unsafeAtomicAdd(a.Lt + (int64_t)pv * a.Ls, pL);
                     ++t_at;
                 }
             }
@@ -276,7 +279,7 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                 defer = false;
                 wcur = wlim = 0;
                 if (ov >= 0) {
-                    unsafeAtomicAdd((float *)&a.AL[ov] + 1, oL);
+                    unsafeAtomicAdd(a.Lt + (int64_t)ov * a.Ls, oL);
                     ++t_at;
                     ov = -1;
                 }
@@ -758,12 +761,13 @@ __global__ __launch_bounds__(256) void k_rec_scatter(const uint2 *__restrict__ r
     }
 }
 
-// Pass 4 (a fixed grid over the chunks): a chunk's records summed in LDS, the sums added to L.
+// Pass 4 (a fixed grid over the chunks): a chunk's records summed in LDS, the sums added to the
+// path-length grid Lt[voxel * Ls].
 __global__ __launch_bounds__(512) void k_rec_accum(const uint2 *__restrict__ sorted,
                                                    const unsigned *__restrict__ tstart,
                                                    const unsigned *__restrict__ cstart,
                                                    const unsigned *__restrict__ cmap, int ntiles, unsigned nvox,
-                                                   float2 *__restrict__ AL)
+                                                   float *__restrict__ Lt, int Ls)
 {
     __shared__ float acc[REC_TILE];
     const unsigned nchunks = cstart[ntiles];
@@ -784,8 +788,9 @@ __global__ __launch_bounds__(512) void k_rec_accum(const uint2 *__restrict__ sor
         for (int q = threadIdx.x; q < REC_TILE; q += 512) {
             const float x = acc[q];
             if (x != 0.f && base + q < nvox) {
-                if (shared_tile) unsafeAtomicAdd((float *)&AL[base + q] + 1, x);
-                else AL[base + q].y += x;
+                float *dst = Lt + (size_t)(base + q) * Ls;
+                if (shared_tile) unsafeAtomicAdd(dst, x);
+                else *dst += x;
             }
         }
         __syncthreads();  // acc is cleared for the next chunk only after every thread has read it
@@ -859,6 +864,7 @@ int join_records(Ctx *c)
 static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a)
 {
     BD_TRY(join_records(c));  // the tallies and record buffers are about to be reused
+    lag_drop(c);
     BD_TRY(launch_pack_optics(c, b->add_plume != 0));
     BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
     BD_TRY(ensure(c, (void **)&c->Q, (size_t)c->nvox * sizeof(float)));
@@ -877,6 +883,8 @@ static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a
     fill_mc_args(c->g, b, a);
     a.opt = c->optics;
     a.AL = c->AL;
+    a.Lt = (float *)c->AL + 1;
+    a.Ls = 2;
     a.acc = c->red + 16;
     a.lat = c->lat;
     a.counter = c->counter;
@@ -937,10 +945,15 @@ int ensure_records(Ctx *c, const bd_beam *b)
         BD_HIP(hipMalloc((void **)&c->rec_off, sizeof(unsigned) * (size_t)REC_SLICES * ntiles));
     }
     if (c->rec_cap >= cap) return BD_OK;
+    // larger buffers: nothing may still use the old ones, and lagged records in them are dropped
+    BD_HIP(hipStreamSynchronize(c->side));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    lag_drop(c);
     if (c->rec) BD_HIP(hipFree(c->rec));
+    if (c->rec_b) BD_HIP(hipFree(c->rec_b));
     if (c->rec_sorted) BD_HIP(hipFree(c->rec_sorted));
     if (c->rec_cmap) BD_HIP(hipFree(c->rec_cmap));
-    c->rec = c->rec_sorted = nullptr;
+    c->rec = c->rec_b = c->rec_sorted = nullptr;
     c->rec_cmap = nullptr;
     c->rec_cap = 0;
     BD_HIP(hipMalloc((void **)&c->rec, sizeof(uint2) * (size_t)cap));
@@ -951,43 +964,93 @@ int ensure_records(Ctx *c, const bd_beam *b)
     return BD_OK;
 }
 
-// The record passes after a deferred transport launch, on stream st.
-static int launch_record_passes(Ctx *c, hipStream_t st)
+// The record passes over record buffer rec (cursor: its slot count) on stream st, adding into
+// the path-length grid Lt[voxel * Ls].
+static int launch_record_passes(Ctx *c, hipStream_t st, const uint2 *rec, const unsigned long long *cursor,
+                                float *Lt, int Ls)
 {
     const int ntiles = (int)((c->nvox + REC_TILE - 1) >> REC_TILE_BITS);
     const size_t lds = sizeof(unsigned) * (size_t)ntiles;
     const unsigned nv = (unsigned)c->nvox;
-    hipLaunchKernelGGL(k_rec_count, dim3(REC_SLICES), dim3(256), lds, st, c->rec, c->counter + 3,
+    hipLaunchKernelGGL(k_rec_count, dim3(REC_SLICES), dim3(256), lds, st, rec, cursor,
                        (long long)c->rec_cap, nv, ntiles, c->rec_tcnt, c->rec_off);
     BD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(1024), 0, st, c->rec_tcnt, ntiles, c->rec_tstart,
                        c->rec_cstart, c->rec_cmap);
     BD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_rec_scatter, dim3(REC_SLICES), dim3(256), lds, st, c->rec, c->counter + 3,
+    hipLaunchKernelGGL(k_rec_scatter, dim3(REC_SLICES), dim3(256), lds, st, rec, cursor,
                        (long long)c->rec_cap, nv, ntiles, c->rec_off, c->rec_tstart, c->rec_sorted);
     BD_HIP(hipGetLastError());
     int ncu = 256;
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     hipLaunchKernelGGL(k_rec_accum, dim3(ncu * 4), dim3(512), 0, st, c->rec_sorted, c->rec_tstart,
-                       c->rec_cstart, c->rec_cmap, ntiles, nv, c->AL);
+                       c->rec_cstart, c->rec_cmap, ntiles, nv, Lt, Ls);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
 
-// The transport launch.  With deferred path lengths, the record passes follow on the context
-// stream, or, with overlap_records, on the side stream, which the caller joins (join_records)
-// before anything reads the path-length tally or the next launch reuses the record buffers.
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, bool overlap_records)
+// The second record buffer and the two path-length grids of lagged launches (eager epochs only,
+// so never inside a graph capture).
+static int ensure_lag_buffers(Ctx *c)
+{
+    if (c->rec_b && c->Lb[0]) return BD_OK;
+    if (!c->rec_b) BD_HIP(hipMalloc((void **)&c->rec_b, sizeof(uint2) * (size_t)c->rec_cap));
+    for (int k = 0; k < 2; ++k)
+        if (!c->Lb[k]) BD_HIP(hipMalloc((void **)&c->Lb[k], sizeof(float) * (size_t)c->nvox));
+    return BD_OK;
+}
+
+int lag_finalize(Ctx *c)
+{
+    BD_TRY(join_records(c));
+    if (c->lag < 0) return BD_OK;
+    const int p = c->lag;
+    c->lag = -1;
+    BD_TRY(launch_record_passes(c, c->stream, p ? c->rec_b : c->rec, c->counter + 3 + p, c->Lb[p], 1));
+    al_set(c, c->Lb[p], 1);
+    return BD_OK;
+}
+
+// The transport launch.  With deferred path lengths the record passes run as rec_mode says:
+// REC_MAIN after the kernel on the context stream; REC_SIDE on the side stream, which the
+// caller joins (join_records) at the end of its epoch; REC_LAG (eager epochs) not in this
+// epoch: this launch writes the record buffer the previous lagged launch did not, and that
+// launch's passes, if still pending, run on the side stream beside this transport.
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, int rec_mode)
 {
     McArgs a;
+    BD_TRY(ensure_records(c, b));  // may drop lagged records (when the buffers grow)
+    const int prev = rec_mode == REC_LAG ? c->lag : -1;
+    lag_drop(c);  // this transport supersedes the L of a lagged launch (its passes still run below)
     BD_TRY(mc_prepare(c, b, ep, a));
-    BD_TRY(ensure_records(c, b));
     const bool defer = c->defer_L && b->score_fluence && c->rec != nullptr;
+    const bool lag = defer && rec_mode == REC_LAG;
+    int cur = 0;
+    if (lag) {
+        BD_TRY(ensure_lag_buffers(c));
+        cur = prev == 0 ? 1 : 0;
+    }
     if (defer) {
-        BD_TRY(launch_fill(c, c->counter + 3, K_F64, 1, 0.0));
-        a.rec = c->rec;
-        a.rec_cursor = c->counter + 3;
+        BD_TRY(launch_fill(c, c->counter + 3 + cur, K_F64, 1, 0.0));
+        a.rec = cur ? c->rec_b : c->rec;
+        a.rec_cursor = c->counter + 3 + cur;
         a.rec_cap = c->rec_cap;
+    }
+    if (lag) {
+        BD_TRY(launch_fill(c, c->Lb[cur], K_F32, c->nvox, 0.0));
+        a.Lt = c->Lb[cur];
+        a.Ls = 1;
+    }
+    if (prev >= 0) {
+        // the previous epoch's passes beside this transport (their L is superseded by this
+        // launch's; they run so that every epoch's tally is reduced)
+        BD_HIP(hipEventRecord(c->ev_fork, c->stream));
+        BD_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        if (timed == 1) BD_HIP(hipEventRecord(c->ev7, c->side));
+        BD_TRY(launch_record_passes(c, c->side, prev ? c->rec_b : c->rec, c->counter + 3 + prev, c->Lb[prev], 1));
+        if (timed == 1) BD_HIP(hipEventRecord(c->ev6, c->side));
+        BD_HIP(hipEventRecord(c->ev_join, c->side));
+        c->join_pending = true;
     }
     const int nblk = (int)mc_blocks(c, b->n_photons);
     if (timed == 1) BD_HIP(hipEventRecord(c->ev0, c->stream));
@@ -996,14 +1059,22 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
     BD_HIP(hipGetLastError());
     if (timed == 1) BD_HIP(hipEventRecord(c->ev1, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[1], c->stream, hipEventRecordExternal));
+    if (lag) {
+        c->lag = cur;
+        if (prev < 0 && timed == 1) {  // no passes this epoch: a zero-length span
+            BD_HIP(hipEventRecord(c->ev7, c->stream));
+            BD_HIP(hipEventRecord(c->ev6, c->stream));
+        }
+        return mc_post(c, b);
+    }
     hipStream_t rs = c->stream;
-    if (defer && overlap_records) {
+    if (defer && rec_mode == REC_SIDE) {
         BD_HIP(hipEventRecord(c->ev_fork, c->stream));
         BD_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         rs = c->side;
     }
     if (timed == 1) BD_HIP(hipEventRecord(c->ev7, rs));
-    if (defer) BD_TRY(launch_record_passes(c, rs));
+    if (defer) BD_TRY(launch_record_passes(c, rs, c->rec, c->counter + 3, (float *)c->AL + 1, 2));
     if (timed == 1) BD_HIP(hipEventRecord(c->ev6, rs));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[6], rs, hipEventRecordExternal));
     if (rs != c->stream) {
@@ -1107,9 +1178,9 @@ int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n)
     return BD_OK;
 }
 
-int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, bool overlap_records)
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, int rec_mode)
 {
-    BD_TRY(launch_mc_enqueue(c, b, nullptr, 1, overlap_records));
+    BD_TRY(launch_mc_enqueue(c, b, nullptr, 1, rec_mode));
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;
@@ -1217,6 +1288,8 @@ extern "C" int bd_mc_run_noalloc(const bd_grid *grid, const bd_beam *beam, const
     fill_mc_args(*grid, beam, a);
     a.opt = (const float4 *)optics;
     a.AL = (float2 *)AL;
+    a.Lt = (float *)AL + 1;
+    a.Ls = 2;
     a.acc = s + NOALLOC_ACC;
     a.lat = s + NOALLOC_LAT;
     a.counter = (unsigned long long *)s;

@@ -546,7 +546,7 @@ int bd_sur_features_mc(bd_ctx *h, const bd_beam *beam, int n, const int32_t *ori
     BD_CHECK(mc_photons >= 0, "bd_sur_features: mc_photons must be >= 0");
     Ctx *c = &h->c;
     BD_CHECK(mc_photons == 0 || c->AL, "bd_sur_features: no MC tallies on the context (run bd_mc_run first)");
-    BD_TRY(join_records(c));
+    BD_TRY(lag_finalize(c));
     BD_TRY(ensure_state(c));
     for (int k = 0; k < n; ++k) BD_TRY(check_origin(c, origins + 4 * k));
     BeamFrame f;
@@ -591,7 +591,7 @@ int bd_sur_targets(bd_ctx *h, const bd_beam *beam, const int32_t *origin, float 
     Ctx *c = &h->c;
     BD_TRY(check_origin(c, origin));
     BD_CHECK(c->AL && c->optics, "bd_sur_targets: no MC tallies on the context (run bd_mc_run first)");
-    BD_TRY(join_records(c));
+    BD_TRY(lag_finalize(c));
     BD_CHECK(beam->n_photons > 0, "bd_sur_targets: n_photons must be > 0");
     BeamFrame f;
     BD_TRY(beam_frame(beam, f));

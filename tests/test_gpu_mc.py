@@ -244,6 +244,70 @@ def test_deferred_path_length_tally_matches_inline(case):
         assert rd.tally_atomics - n_a > 0.5 * n_l  # the rest went inline
 
 
+def _engine_pair(**over):
+    """Two engines on the same small layered stack and seeds: one with the path-length tally
+    deferred to records (the default), one adding it inline."""
+    import os
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.workloads import layered_stack
+    kw = layered_stack(shape=(40, 40, 56), n_photons=30_000, plume_grid=None, seed=5, power=2.0)
+    kw.update(over)
+    out = []
+    for defer in (True, False):
+        old = os.environ.get("BUGDEF_MC_DEFER_FLUENCE")
+        os.environ["BUGDEF_MC_DEFER_FLUENCE"] = "1" if defer else "0"
+        try:
+            out.append(CoupledAblationEngine(**kw))
+        finally:
+            if old is None:
+                del os.environ["BUGDEF_MC_DEFER_FLUENCE"]
+            else:
+                os.environ["BUGDEF_MC_DEFER_FLUENCE"] = old
+    return out
+
+
+def _tallies_close(ed, ei):
+    for name in ('A', 'L'):
+        gd, gi = ed._ctx.download(name), ei._ctx.download(name)
+        assert gi.sum() > 0.0, name
+        assert gd.sum() == pytest.approx(gi.sum(), rel=1e-5), name
+        assert np.all(np.abs(gd - gi) <= 2e-4 * np.abs(gi) + 1e-7 * gi.max()), name
+
+
+@pytest.mark.parametrize("schedule", ["read_every_epoch", "read_after_two", "no_transport_epoch_between"])
+def test_epoch_path_length_tally_lags_one_epoch_and_is_complete_on_read(schedule):
+    """Eager epochs leave their path-length records to the next epoch, which reduces them beside
+    its own transport; any read of the tally first completes the latest transport's.  Against an
+    engine that adds path lengths inline (same photons, low power so no cell changes phase): the
+    A and L tallies read right after an epoch, after two epochs with no read between (the first
+    epoch's records reduced during the second), and after an epoch without transport (which keeps
+    the last transport's tally), up to the order of the float additions."""
+    ed, ei = _engine_pair()
+    try:
+        if schedule == "read_every_epoch":
+            for _ in range(3):
+                ed.step_epoch()
+                ei.step_epoch()
+                _tallies_close(ed, ei)
+        elif schedule == "read_after_two":
+            for _ in range(2):
+                ed.step_epoch()
+                ei.step_epoch()
+            assert ed.last_diag.t_mc_records_ms > 0.0  # the first epoch's passes ran in the second
+            _tallies_close(ed, ei)
+        else:
+            ed.step_epoch()
+            ei.step_epoch()
+            ed.step_epoch_with_q()
+            ei.step_epoch_with_q()
+            _tallies_close(ed, ei)
+            assert np.asarray(ed.Phi).max() > 0.0
+        assert not np.any(np.asarray(ed.ablated) != np.asarray(ei.ablated))
+    finally:
+        ed.close()
+        ei.close()
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 1), (1, 1, 6), (3, 2, 1), (2, 5, 3)])
 def test_degenerate_grids_gpu_vs_oracle(oracle, shape):
     """One voxel, one column, one plane: the entrance split, exits through every
