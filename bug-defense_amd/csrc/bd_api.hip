@@ -145,6 +145,8 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     BD_HIP(hipEventCreate(&c->ev5));
     BD_HIP(hipEventCreate(&c->ev6));
     BD_HIP(hipEventCreate(&c->ev7));
+    BD_HIP(hipEventCreate(&c->ev8));
+    BD_HIP(hipEventCreate(&c->ev9));
     BD_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     BD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     BD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
@@ -200,6 +202,8 @@ int bd_destroy(bd_ctx *h)
     if (c->ev5) (void)hipEventDestroy(c->ev5);
     if (c->ev6) (void)hipEventDestroy(c->ev6);
     if (c->ev7) (void)hipEventDestroy(c->ev7);
+    if (c->ev8) (void)hipEventDestroy(c->ev8);
+    if (c->ev9) (void)hipEventDestroy(c->ev9);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->side) (void)hipStreamDestroy(c->side);

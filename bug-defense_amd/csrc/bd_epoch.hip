@@ -56,44 +56,25 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
     const double dt_epoch = n_substeps * p.dt_sub;
     const bool plume = c->plume.enabled != 0;
     const bool diff_plume = plume && c->plume.model == 0;
+    // The epoch is queued without a host wait until its end (one readback of the scalars), except
+    // with the blast or separable plume, whose injection and update take the count of removed
+    // cells on the host.
+    const bool host_plume = plume && (c->plume.model != 0 || c->plume.blast_expansion);
+    const bool mc = beam->power > 0.0 && run_mc;
     hipEvent_t e_start = c->ev2, e_opt = c->ev3;
 
     // ---- 1. optics
     BD_HIP(hipEventRecord(e_start, c->stream));
     if (beam->power > 0.0 && !run_mc) {
-        double qsum = 0.0;
-        BD_TRY(reduce_masked(c, c->Q, c->nvox, c->flags, F_ABL, true, &qsum));
-        out->absorbed_J = qsum * V * dt_epoch;
-        out->absorbed_frac = qsum * V / beam->power;
-        out->mc_energy_closure = std::nan("");
-    } else if (beam->power > 0.0) {
-        bd_mc_result mc{};
-        BD_TRY(launch_mc(c, beam, &mc, REC_LAG));  // this epoch's record passes run at the next one
-        const double N = (double)beam->n_photons;
-        out->absorbed_frac = mc.absorbed / N;
-        out->mc_energy_closure = (mc.specular + mc.diffuse + mc.transmitted + mc.lateral + mc.absorbed) / N;
-        out->reflected_J = (mc.specular + mc.diffuse) / N * beam->power * dt_epoch;
-        out->transmitted_J = mc.transmitted / N * beam->power * dt_epoch;
-        out->lateral_J = mc.lateral / N * beam->power * dt_epoch;
-        out->R_spec = mc.specular / N;
-        out->R_diff = mc.diffuse / N;
-        out->transmission = mc.transmitted / N;
-        out->lateral_loss = mc.lateral / N;
-        out->mc_events = mc.events;
-        out->mc_tally_atomics = mc.tally_atomics;
-        out->mc_tally_records = mc.tally_records;
-        out->t_mc_kernel_ms = mc.kernel_ms;
-        out->t_mc_records_ms = mc.record_ms;  // NaN while the previous epoch's run beside the transport
-        double qsum = 0.0;
-        BD_TRY(reduce_masked(c, c->Q, c->nvox, c->flags, F_ABL, true, &qsum));
-        out->absorbed_J = qsum * V * dt_epoch;
+        BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
+    } else if (mc) {
+        BD_TRY(launch_mc_enqueue(c, beam, nullptr, 1, REC_LAG));  // this epoch's record passes run at the next one
+        BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
     } else {
         BD_TRY(join_records(c));
         lag_drop(c);  // the cleared tally supersedes a lagged launch's
         BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
         if (c->AL) BD_TRY(launch_fill(c, c->AL, K_F32, 2 * c->nvox, 0.0));
-        out->absorbed_frac = 0.0;
-        out->mc_energy_closure = 1.0;
     }
     BD_HIP(hipEventRecord(e_opt, c->stream));
 
@@ -112,23 +93,64 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         const double decay = (tl > 0.0 && std::isfinite(tl)) ? std::exp(-p.dt_sub / tl) : 1.0;
         BD_TRY(launch_plume_substeps(c, n_substeps, p.dt_sub, decay));
     }
-    BD_HIP(hipEventRecord(c->ev0, c->stream));
+    BD_HIP(hipEventRecord(c->ev8, c->stream));
 
     // ---- 3. phase update + removal
-    int64_t n_new = 0;
-    double abl_J = 0.0;
-    BD_TRY(launch_phase(c, t_after, &n_new, &abl_J));
-    out->n_new = n_new;
-    out->ablated_J = abl_J;
+    int64_t n_new_host = 0;
+    if (host_plume) BD_TRY(launch_phase(c, t_after, &n_new_host, nullptr));
+    else BD_TRY(launch_phase_enqueue(c, t_after, nullptr));
 
     // ---- 4. plume
     if (plume) {
         BD_TRY(launch_plume_void(c));  // void after removal (coupled_ablation.py:588,658)
-        if (n_new > 0) BD_TRY(launch_plume_inject(c));
-        if (diff_plume) BD_TRY(launch_plume_sync(c));
-        else BD_TRY(launch_plume_update_sep(c, dt_epoch));
+        if (!host_plume) {
+            BD_TRY(launch_plume_inject(c));  // point injection: nothing is added where nothing was removed
+            BD_TRY(launch_plume_sync(c));
+        } else {
+            if (n_new_host > 0) BD_TRY(launch_plume_inject(c));
+            if (diff_plume) BD_TRY(launch_plume_sync(c));
+            else BD_TRY(launch_plume_update_sep(c, dt_epoch));
+        }
+        BD_TRY(launch_plume_diag_enqueue(c));
+    }
+    BD_TRY(join_records(c));
+    BD_HIP(hipEventRecord(c->ev9, c->stream));
+    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red, 48 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipMemcpyAsync((char *)c->host_pinned + 48 * sizeof(double), c->counter, 2 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(hipStreamSynchronize(c->stream));
+    const double *r = (const double *)c->host_pinned;
+    const unsigned long long *cnt = (const unsigned long long *)(r + 48);
+    if (beam->power > 0.0 && !run_mc) {
+        out->absorbed_J = r[40] * V * dt_epoch;
+        out->absorbed_frac = r[40] * V / beam->power;
+        out->mc_energy_closure = std::nan("");
+    } else if (mc) {
+        const double N = (double)beam->n_photons;
+        const double *m = r + 16;  // the transport's FP64 scalar tallies
+        out->absorbed_frac = m[4] / N;
+        out->mc_energy_closure = (m[0] + m[1] + m[2] + m[3] + m[4]) / N;
+        out->reflected_J = (m[0] + m[1]) / N * beam->power * dt_epoch;
+        out->transmitted_J = m[2] / N * beam->power * dt_epoch;
+        out->lateral_J = m[3] / N * beam->power * dt_epoch;
+        out->R_spec = m[0] / N;
+        out->R_diff = m[1] / N;
+        out->transmission = m[2] / N;
+        out->lateral_loss = m[3] / N;
+        out->mc_events = m[5];
+        out->mc_tally_atomics = m[6];
+        out->mc_tally_records = m[7];
+        out->absorbed_J = r[40] * V * dt_epoch;
+    } else {
+        out->absorbed_frac = 0.0;
+        out->mc_energy_closure = 1.0;
+    }
+    out->conducted_J = -r[8];
+    out->ablated_J = r[9];
+    out->n_new = (int64_t)cnt[1];
+    if (plume) {
         double d[4];
-        BD_TRY(launch_plume_diag(c, d));
+        plume_diag_from(r + 24, d);
         out->plume_tau = d[0];
         out->plume_transmission = d[1];
         out->plume_peak_mu_a = d[2];
@@ -137,22 +159,18 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
         out->plume_tau = 0.0;
         out->plume_transmission = 1.0;
     }
-    const bool records_beside = c->join_pending;
-    BD_TRY(join_records(c));
-    BD_HIP(hipEventRecord(c->ev1, c->stream));
-    BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    BD_HIP(hipStreamSynchronize(c->stream));
-    out->conducted_J = -*(double *)c->host_pinned;
     float ms = 0.f;
-    if (records_beside) {
-        BD_HIP(hipEventElapsedTime(&ms, c->ev7, c->ev6));
+    if (mc) {
+        BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));  // recorded around the transport kernel
+        out->t_mc_kernel_ms = ms;
+        BD_HIP(hipEventElapsedTime(&ms, c->ev7, c->ev6));  // the record passes of the previous epoch (0: none)
         out->t_mc_records_ms = ms;
     }
     BD_HIP(hipEventElapsedTime(&ms, e_start, e_opt));
     out->t_optical_ms = ms;
-    BD_HIP(hipEventElapsedTime(&ms, e_opt, c->ev0));
+    BD_HIP(hipEventElapsedTime(&ms, e_opt, c->ev8));
     out->t_thermal_ms = ms;
-    BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    BD_HIP(hipEventElapsedTime(&ms, c->ev8, c->ev9));
     out->t_phase_ms = ms;
     BD_HIP(hipEventElapsedTime(&ms, c->ev4, c->ev5));
     out->t_substeps_ms = ms;

@@ -51,6 +51,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
     hipEvent_t ev6 = nullptr, ev7 = nullptr;  // end / start of the path-length record passes
+    hipEvent_t ev8 = nullptr, ev9 = nullptr;  // eager epoch: end of the thermal phase, end of the epoch
     // the record passes of an epoch's transport run on `side`, beside the thermal sub-steps
     // (nothing in the epoch reads the path-length tally); join_records() makes `stream` wait
     hipStream_t side = nullptr;
@@ -201,6 +202,7 @@ int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n);
 int launch_phase(Ctx *c, double t_now, int64_t *n_new, double *ablated_J);
 int launch_phase_enqueue(Ctx *c, double t_now, const EpochParams *ep);
 int launch_plume_diag_enqueue(Ctx *c);
+void plume_diag_from(const double *h, double out[4]);  // tau, transmission, peak, mean from the accumulators
 int reduce_masked_enqueue(Ctx *c, const float *src, int64_t n, const uint8_t *flags, uint8_t bit, bool invert,
                           double *dev_out);
 int launch_plume_void(Ctx *c);
