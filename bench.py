@@ -234,6 +234,9 @@ def main():
                          "over a range of address spans (for profiling the MC kernel)")
     args = ap.parse_args()
     if args.mc_probe:
+        # launches of their own add path lengths inline unless told otherwise; the probe times the
+        # kernel as the epochs run it, with the path lengths deferred to records
+        os.environ.setdefault("BUGDEF_MC_DEFER_FLUENCE", "1")
         from simulation.coupled_ablation import CoupledAblationEngine
         from simulation.workloads import layered_stack
         eng = CoupledAblationEngine(**layered_stack())

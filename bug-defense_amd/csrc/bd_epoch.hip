@@ -231,7 +231,6 @@ static int enqueue_epoch_tail(Ctx *c, int n_substeps)
         BD_TRY(launch_plume_sync(c));
         BD_TRY(launch_plume_diag_enqueue(c));
     }
-    BD_TRY(join_records(c));  // the record passes ran beside the sub-steps
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red, 48 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipMemcpyAsync((char *)c->host_pinned + 48 * sizeof(double), c->counter, 2 * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, c->stream));
@@ -240,7 +239,10 @@ static int enqueue_epoch_tail(Ctx *c, int n_substeps)
 
 static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
 {
-    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2, REC_SIDE));
+    // path lengths inline by default (records only with BUGDEF_MC_DEFER_FLUENCE=1, their passes
+    // then right after the transport on the context stream: with many contexts' captured epochs
+    // in flight, a second stream per context cost more than it overlapped)
+    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2, REC_MAIN));
     BD_TRY(enqueue_after_mc(c, beam));
     return enqueue_epoch_tail(c, n_substeps);
 }
@@ -326,7 +328,7 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
         BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
         BD_TRY(ensure(c, (void **)&c->optics, (size_t)c->nvox * sizeof(float4)));
         BD_TRY(ensure(c, (void **)&c->hlat, (size_t)c->nvox * 4));
-        BD_TRY(ensure_records(c, beam));
+        BD_TRY(ensure_records(c, beam, REC_MAIN));
         const int64_t nd = beam->launch_axis == 0 ? c->g.nz : c->g.nx;
         if (c->lat_n != nd) {
             if (c->lat) BD_HIP(hipFree(c->lat));

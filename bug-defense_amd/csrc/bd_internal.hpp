@@ -52,8 +52,10 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
     hipEvent_t ev6 = nullptr, ev7 = nullptr;  // end / start of the path-length record passes
     hipEvent_t ev8 = nullptr, ev9 = nullptr;  // eager epoch: end of the thermal phase, end of the epoch
-    // the record passes of an epoch's transport run on `side`, beside the thermal sub-steps
-    // (nothing in the epoch reads the path-length tally); join_records() makes `stream` wait
+    // an eager epoch runs the previous epoch's record passes on `side`, beside its transport;
+    // join_records() makes `stream` wait.  Created with the lag buffers, on first use: with
+    // few hardware queues per process (4 by default) a stream per context that most contexts
+    // never use takes queues from the context streams of concurrent runs.
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool join_pending = false;
@@ -109,7 +111,9 @@ struct Ctx {
     uint2 *rec_b = nullptr;
     float *Lb[2] = {nullptr, nullptr};
     int lag = -1;
-    bool defer_L = true;                 // BUGDEF_MC_DEFER_FLUENCE=0: the transport adds path lengths inline
+    // path lengths to records: 2 (default) only in eager epochs, where their passes run beside the
+    // next transport; 1 in every launch; 0 never (BUGDEF_MC_DEFER_FLUENCE=1 / 0)
+    int defer_L = 2;
     int rec_per_photon = 512;            // record buffer size per photon (BUGDEF_MC_RECORDS_PER_PHOTON)
     void *staging = nullptr;             // f64 staging for conversions
     size_t staging_bytes = 0;
@@ -174,7 +178,7 @@ void set_error(const std::string &msg);
 int ensure(Ctx *c, void **p, size_t bytes);  // lazily allocate (and zero) a buffer
 int ensure_state(Ctx *c);                     // T, T2, H, chi, Q, k, rc, flags, ...
 int ensure_staging(Ctx *c, size_t bytes);
-int ensure_records(Ctx *c, const bd_beam *b);  // the transport's path-length record buffer (when deferred)
+int ensure_records(Ctx *c, const bd_beam *b, int rec_mode);  // record buffers, when such a launch defers
 
 // launch helpers implemented in the kernel translation units
 int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger);
@@ -182,10 +186,10 @@ int launch_plume_substeps(Ctx *c, int n, double dt, double decay);
 int launch_plateau(Ctx *c);
 int launch_arrhenius(Ctx *c, double A, double EaR, double dt);
 int launch_char(Ctx *c, double A, double EaR, double dt);
-// where the record passes of a transport launch run: after it on the context stream, on the
-// side stream beside the rest of the epoch (joined at its end), or at the next eager epoch
-enum RecMode { REC_MAIN = 0, REC_SIDE = 1, REC_LAG = 2 };
-int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, int rec_mode = REC_MAIN);
+// where the record passes of a transport launch run: after it on the context stream, or at the
+// next eager epoch, on the side stream beside that epoch's transport
+enum RecMode { REC_MAIN = 0, REC_LAG = 2 };
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out);  // bd_mc_run: passes after the kernel
 int join_records(Ctx *c);   // the context stream waits for record passes left on the side stream
 int lag_finalize(Ctx *c);   // the lagged record passes (if any) run and their L lands in AL
 void al_get(Ctx *c, float *d, int which);        // one half of AL (0: A, 1: L) into a float grid

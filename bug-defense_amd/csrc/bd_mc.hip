@@ -934,10 +934,22 @@ static int mc_post(Ctx *c, const bd_beam *b)
 // default; a photon of the layered-stack workloads leaves 150-400), between 2^16 and 2^28
 // records (0.5 MB - 2 GB).  A launch that fills it adds the rest inline, so the size bounds only
 // how much is deferred.
-int ensure_records(Ctx *c, const bd_beam *b)
+// Whether a launch of mode rec_mode writes path-length records.  By default only lagged launches
+// do: their passes run beside the next transport.  After a launch of its own or inside a captured
+// epoch the passes cost more than the inline atomics they replace wherever photons are mostly
+// scattering events (the campaign's 320-532 nm runs: 1 900 events per photon, the kernel time the
+// same with and without records, the passes 1.2-1.7 ms more per epoch).
+static bool defers(const Ctx *c, const bd_beam *b, int rec_mode)
 {
     const int64_t ntiles = (c->nvox + REC_TILE - 1) >> REC_TILE_BITS;
-    if (!c->defer_L || !b->score_fluence || ntiles > REC_MAX_TILES) return BD_OK;
+    if (!b->score_fluence || ntiles > REC_MAX_TILES || c->defer_L == 0) return false;
+    return c->defer_L == 1 || rec_mode == REC_LAG;
+}
+
+int ensure_records(Ctx *c, const bd_beam *b, int rec_mode)
+{
+    const int64_t ntiles = (c->nvox + REC_TILE - 1) >> REC_TILE_BITS;
+    if (!defers(c, b, rec_mode)) return BD_OK;
     int64_t cap = (int64_t)b->n_photons * c->rec_per_photon;
     cap = std::min<int64_t>(std::max<int64_t>(cap, int64_t(1) << 16), int64_t(1) << 28);
     if (!c->rec_tcnt) {
@@ -949,7 +961,7 @@ int ensure_records(Ctx *c, const bd_beam *b)
     }
     if (c->rec_cap >= cap) return BD_OK;
     // larger buffers: nothing may still use the old ones, and lagged records in them are dropped
-    BD_HIP(hipStreamSynchronize(c->side));
+    if (c->side) BD_HIP(hipStreamSynchronize(c->side));
     BD_HIP(hipStreamSynchronize(c->stream));
     lag_drop(c);
     if (c->rec) BD_HIP(hipFree(c->rec));
@@ -996,7 +1008,8 @@ static int launch_record_passes(Ctx *c, hipStream_t st, const uint2 *rec, const 
 // so never inside a graph capture).
 static int ensure_lag_buffers(Ctx *c)
 {
-    if (c->rec_b && c->Lb[0]) return BD_OK;
+    if (c->rec_b && c->Lb[0] && c->side) return BD_OK;
+    if (!c->side) BD_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     if (!c->rec_b) BD_HIP(hipMalloc((void **)&c->rec_b, sizeof(uint2) * (size_t)c->rec_cap));
     for (int k = 0; k < 2; ++k)
         if (!c->Lb[k]) BD_HIP(hipMalloc((void **)&c->Lb[k], sizeof(float) * (size_t)c->nvox));
@@ -1015,18 +1028,17 @@ int lag_finalize(Ctx *c)
 }
 
 // The transport launch.  With deferred path lengths the record passes run as rec_mode says:
-// REC_MAIN after the kernel on the context stream; REC_SIDE on the side stream, which the
-// caller joins (join_records) at the end of its epoch; REC_LAG (eager epochs) not in this
-// epoch: this launch writes the record buffer the previous lagged launch did not, and that
-// launch's passes, if still pending, run on the side stream beside this transport.
+// REC_MAIN after the kernel on the context stream; REC_LAG (eager epochs) not in this epoch:
+// this launch writes the record buffer the previous lagged launch did not, and that launch's
+// passes, if still pending, run on the side stream beside this transport.
 int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, int rec_mode)
 {
     McArgs a;
-    BD_TRY(ensure_records(c, b));  // may drop lagged records (when the buffers grow)
+    BD_TRY(ensure_records(c, b, rec_mode));  // may drop lagged records (when the buffers grow)
     const int prev = rec_mode == REC_LAG ? c->lag : -1;
     lag_drop(c);  // this transport supersedes the L of a lagged launch (its passes still run below)
     BD_TRY(mc_prepare(c, b, ep, a));
-    const bool defer = c->defer_L && b->score_fluence && c->rec != nullptr;
+    const bool defer = defers(c, b, rec_mode) && c->rec != nullptr;
     const bool lag = defer && rec_mode == REC_LAG;
     int cur = 0;
     if (lag) {
@@ -1070,20 +1082,10 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
         }
         return mc_post(c, b);
     }
-    hipStream_t rs = c->stream;
-    if (defer && rec_mode == REC_SIDE) {
-        BD_HIP(hipEventRecord(c->ev_fork, c->stream));
-        BD_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        rs = c->side;
-    }
-    if (timed == 1) BD_HIP(hipEventRecord(c->ev7, rs));
-    if (defer) BD_TRY(launch_record_passes(c, rs, c->rec, c->counter + 3, (float *)c->AL + 1, 2));
-    if (timed == 1) BD_HIP(hipEventRecord(c->ev6, rs));
-    if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[6], rs, hipEventRecordExternal));
-    if (rs != c->stream) {
-        BD_HIP(hipEventRecord(c->ev_join, rs));
-        c->join_pending = true;
-    }
+    if (timed == 1) BD_HIP(hipEventRecord(c->ev7, c->stream));
+    if (defer) BD_TRY(launch_record_passes(c, c->stream, c->rec, c->counter + 3, (float *)c->AL + 1, 2));
+    if (timed == 1) BD_HIP(hipEventRecord(c->ev6, c->stream));
+    if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[6], c->stream, hipEventRecordExternal));
     return mc_post(c, b);
 }
 
@@ -1181,9 +1183,9 @@ int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n)
     return BD_OK;
 }
 
-int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, int rec_mode)
+int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out)
 {
-    BD_TRY(launch_mc_enqueue(c, b, nullptr, 1, rec_mode));
+    BD_TRY(launch_mc_enqueue(c, b, nullptr, 1, REC_MAIN));
     BD_HIP(hipMemcpyAsync(c->host_pinned, c->red + 16, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(hipStreamSynchronize(c->stream));
     const double *r = (const double *)c->host_pinned;
@@ -1198,12 +1200,8 @@ int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out, int rec_mode)
     float ms = 0.f;
     BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     out->kernel_ms = ms;  // the transport kernel alone
-    if (c->join_pending) {
-        out->record_ms = std::nan("");  // still running on the side stream; the caller times it after joining
-    } else {
-        BD_HIP(hipEventElapsedTime(&ms, c->ev7, c->ev6));
-        out->record_ms = ms;  // the path-length record passes (0 when the transport adds them inline)
-    }
+    BD_HIP(hipEventElapsedTime(&ms, c->ev7, c->ev6));
+    out->record_ms = ms;  // the path-length record passes (0 when the transport adds them inline)
     return BD_OK;
 }
 
