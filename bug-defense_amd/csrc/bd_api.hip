@@ -153,7 +153,11 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     if (const char *e = std::getenv("BUGDEF_MC_BLOCKS_PER_CU")) c->mc_blocks_per_cu = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("BUGDEF_MC_DEFER_FLUENCE")) c->defer_L = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("BUGDEF_MC_RECORDS_PER_PHOTON")) c->rec_per_photon = std::max(1, std::atoi(e));
-    if (const char *e = std::getenv("BUGDEF_SUBSTEP_PAIRS")) c->two_step_allowed = std::atoi(e) != 0;
+    if (const char *e = std::getenv("BUGDEF_SUBSTEP_PAIRS")) {
+        c->two_step_allowed = std::atoi(e) != 0;
+        c->two_step_forced = std::atoi(e) == 2;
+    }
+    if (const char *e = std::getenv("BUGDEF_TUNE_LOG")) c->tune_log = std::atoi(e) != 0;
     // a fixed one-step tile "BYxXC" (BY 2, 4 or 8; XC >= 1) in place of the timed choice, e.g. for
     // counter runs, where the profiler's per-dispatch overhead distorts the tuning launches
     if (const char *e = std::getenv("BUGDEF_SUBSTEP_TILE")) {

@@ -127,6 +127,8 @@ struct Ctx {
     int tile2_by = 16, tile2_xc = 32;
     bool use_two = false;
     bool two_step_allowed = true;      // BUGDEF_SUBSTEP_PAIRS=0 keeps every sub-step in its own launch
+    bool two_step_forced = false;      // BUGDEF_SUBSTEP_PAIRS=2 fuses pairs whatever the timing pass finds
+    bool tune_log = false;             // BUGDEF_TUNE_LOG=1 prints the timing pass's table to stderr
     float *Hb = nullptr;               // second H_acc buffer (the two-sub-step kernel writes the other one)
     uint8_t *flagsb = nullptr;         // second flags buffer
     bool tuned = false;

@@ -17,6 +17,7 @@
 #include "bd_internal.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <utility>
 
 namespace bd {
@@ -54,7 +55,6 @@ struct SubArgs {
 };
 
 
-__device__ __forceinline__ float hmean(float a, float b) { return 2.0f * a * b / (a + b + 1e-12f); }
 
 // Face conductance and 1/rho_cp with the hardware reciprocal (v_rcp_f32, 1 ulp):
 // the stencil's divisions dominate its ALU time, and the reference evaluates
@@ -84,38 +84,27 @@ __device__ __forceinline__ StepK step_consts(const SubArgs &a)
     return k;
 }
 
-// One explicit sub-step of one cell (voxel_thermal_3d.py:30-143): face fluxes in
-// the reference's order (w, e, s, n, top, bottom), source, the latent-heat
-// plateau on (h, fl).  Both kernels evaluate every cell through this function
-// with contraction off, so the two paths produce the same bits.  Returns the
-// new T; *fbound gets the boundary flux of the cell.
-__device__ __forceinline__ float cell_step(const StepK &a, int x, int y, int z, float Tc, float kc, float Tm, float km,
-                                           float Tp, float kp, float Ts, float ks, float Tn_, float kn, float Tt,
-                                           float kt, float Tb, float kb, float cap, float qv, uint8_t fl, float &h,
-                                           uint8_t &fl2, float *fbound)
+// The series-resistance flux of a top or bottom boundary cell (voxel_thermal_3d.py:84-98), with
+// the hardware reciprocal (1 ulp) like the face conductances: the two IEEE divisions it replaces
+// were issued on every plane by each wave holding a boundary lane.
+__device__ __forceinline__ float bnd_flux(const StepK &a, float Tc, float kc)
 {
 #pragma clang fp contract(off)
-    float fw = 0.f, fe = 0.f, fs = 0.f, fn = 0.f, ft = 0.f, fb = 0.f;
-    if (x > 0) fw = hmean_fast(kc, km) * (Tm - Tc) * a.idx2;
-    if (x < a.Nx - 1) fe = hmean_fast(kc, kp) * (Tp - Tc) * a.idx2;
-    if (y > 0) fs = hmean_fast(kc, ks) * (Ts - Tc) * a.idy2;
-    if (y < a.Ny - 1) fn = hmean_fast(kc, kn) * (Tn_ - Tc) * a.idy2;
-    float fbd = 0.f;
-    if (z > 0) ft = hmean_fast(kc, kt) * (Tt - Tc) * a.idz2;
-    else if (a.h_eff > 0.f) {
-        ft = (a.t_amb - Tc) / (a.dz * (0.5f * a.dz / fmaxf(kc, 1e-30f) + a.inv_h));
-        fbd += ft;
-    }
-    if (z < a.Nz - 1) fb = hmean_fast(kc, kb) * (Tb - Tc) * a.idz2;
-    else if (a.h_eff > 0.f) {
-        fb = (a.t_amb - Tc) / (a.dz * (0.5f * a.dz / fmaxf(kc, 1e-30f) + a.inv_h));
-        fbd += fb;
-    }
-    const float div = fw + fe + fs + fn + ft + fb;
+    return (a.t_amb - Tc) *
+           __builtin_amdgcn_rcpf(a.dz * (0.5f * a.dz * __builtin_amdgcn_rcpf(fmaxf(kc, 1e-30f)) + a.inv_h));
+}
+
+// Source, explicit update and latent-heat plateau of one cell from its face-flux sum `div`
+// (voxel_thermal_3d.py:100-143) on (h, fl).  Every sub-step kernel ends each cell here, with
+// contraction off, so they all produce the same bits.
+__device__ __forceinline__ float cell_tail(const StepK &a, float Tc, float div, float cap, float qv, uint8_t fl,
+                                           float &h, uint8_t &fl2)
+{
+#pragma clang fp contract(off)
     float src = qv * a.qscale;
     if (a.maskq && (fl & F_ABL)) src = 0.f;
-    float Tn = Tc + a.dt * (div + src) * __builtin_amdgcn_rcpf(cap);
-    *fbound = fbd;
+    const float icap = __builtin_amdgcn_rcpf(cap);
+    float Tn = Tc + a.dt * (div + src) * icap;
     fl2 = fl;
     if (a.plateau) {
         if (fl & F_ABL) {
@@ -132,11 +121,41 @@ __device__ __forceinline__ float cell_step(const StepK &a, int x, int y, int z, 
             const float rel = fminf(hh, (a.t_abl - Tn) * cap);
             hh -= rel;
             h = hh;
-            Tn = Tn + rel / cap;
+            Tn = Tn + rel * icap;
             if (!(hh > 0.f)) fl2 = (uint8_t)(fl & ~F_HPOS);
         }
     }
     return Tn;
+}
+
+// One explicit sub-step of one cell (voxel_thermal_3d.py:30-143): face fluxes in
+// the reference's order (w, e, s, n, top, bottom), source, the latent-heat
+// plateau on (h, fl).  Returns the new T; *fbound gets the boundary flux of the cell.
+__device__ __forceinline__ float cell_step(const StepK &a, int x, int y, int z, float Tc, float kc, float Tm, float km,
+                                           float Tp, float kp, float Ts, float ks, float Tn_, float kn, float Tt,
+                                           float kt, float Tb, float kb, float cap, float qv, uint8_t fl, float &h,
+                                           uint8_t &fl2, float *fbound)
+{
+#pragma clang fp contract(off)
+    float fw = 0.f, fe = 0.f, fs = 0.f, fn = 0.f, ft = 0.f, fb = 0.f;
+    if (x > 0) fw = hmean_fast(kc, km) * (Tm - Tc) * a.idx2;
+    if (x < a.Nx - 1) fe = hmean_fast(kc, kp) * (Tp - Tc) * a.idx2;
+    if (y > 0) fs = hmean_fast(kc, ks) * (Ts - Tc) * a.idy2;
+    if (y < a.Ny - 1) fn = hmean_fast(kc, kn) * (Tn_ - Tc) * a.idy2;
+    float fbd = 0.f;
+    if (z > 0) ft = hmean_fast(kc, kt) * (Tt - Tc) * a.idz2;
+    else if (a.h_eff > 0.f) {
+        ft = bnd_flux(a, Tc, kc);
+        fbd += ft;
+    }
+    if (z < a.Nz - 1) fb = hmean_fast(kc, kb) * (Tb - Tc) * a.idz2;
+    else if (a.h_eff > 0.f) {
+        fb = bnd_flux(a, Tc, kc);
+        fbd += fb;
+    }
+    const float div = fw + fe + fs + fn + ft + fb;
+    *fbound = fbd;
+    return cell_tail(a, Tc, div, cap, qv, fl, h, fl2);
 }
 
 // Char kinetics (coupled_ablation.py:48-58) run on dry, intact cells above
@@ -152,7 +171,7 @@ __device__ __forceinline__ bool char_active(const StepK &a, float Tn, uint8_t fl
 __device__ __forceinline__ float char_step(const StepK &a, float Tn, float chi)
 {
 #pragma clang fp contract(off)
-    const float rate = expf(a.chLnA - a.chEaR / (Tn + 273.15f));
+    const float rate = expf(a.chLnA - a.chEaR * __builtin_amdgcn_rcpf(Tn + 273.15f));
     return 1.0f - (1.0f - chi) * expf(-rate * a.dt);
 }
 
@@ -160,7 +179,7 @@ __device__ __forceinline__ float char_step(const StepK &a, float Tn, float chi)
 __device__ __forceinline__ float arr_step(const StepK &a, float Tn, float om)
 {
 #pragma clang fp contract(off)
-    return om + a.dt * expf(a.arrLnA - a.arrEaR / (Tn + 273.15f));
+    return om + a.dt * expf(a.arrLnA - a.arrEaR * __builtin_amdgcn_rcpf(Tn + 273.15f));
 }
 
 // Arrhenius damage and char kinetics of one cell at its new temperature
@@ -679,6 +698,7 @@ static int autotune(Ctx *c)
             BD_HIP(hipEventRecord(c->ev1, c->stream));
             BD_HIP(hipEventSynchronize(c->ev1));
             BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            if (c->tune_log) std::fprintf(stderr, "tune pair %dx%d: %.1f us\n", by, xc, ms * 1000.f / 3.f);
             if (ms < best2) {
                 best2 = ms;
                 bb2 = by;
@@ -687,7 +707,15 @@ static int autotune(Ctx *c)
         }
     c->tile2_by = bb2;
     c->tile2_xc = bx2;
-    c->use_two = c->two_step_allowed && best2 < 2.f * best;  // three one-step vs three two-step launches
+    // three one-step vs three two-step launches
+    c->use_two = c->two_step_allowed && (c->two_step_forced || best2 < 2.f * best);
+    if (c->tune_log) {
+        for (int ib = 0; ib < 3; ++ib)
+            for (int ix = 0; ix < 4; ++ix)
+                std::fprintf(stderr, "tune %dx%d: %.1f us\n", bys[ib], xcs[ix], tms[ib][ix] * 1000.f / 3.f);
+        std::fprintf(stderr, "tune pair best %dx%d: %.1f us per launch (two sub-steps)%s\n", bb2, bx2,
+                     best2 * 1000.f / 3.f, c->use_two ? ", used" : "");
+    }
     if (c->use_two) {
         // the second H / flags buffers exist before any epoch is captured
         BD_TRY(ensure(c, (void **)&c->Hb, (size_t)c->nvox * 4));
@@ -770,7 +798,7 @@ __global__ void k_plateau(float *__restrict__ T, float *__restrict__ H, const fl
             const float rel = fminf(h, (t_abl - t) * cap);
             h -= rel;
             H[i] = h;
-            T[i] = t + rel / cap;
+            T[i] = t + rel * __builtin_amdgcn_rcpf(cap);
             if (!(h > 0.f)) flags[i] = (uint8_t)(fl & ~F_HPOS);
         }
     }
@@ -780,7 +808,7 @@ __global__ void k_arrhenius(const float *__restrict__ T, float *__restrict__ Om,
                             float lnA, float EaR, float dt, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        if (!(flags[i] & F_ABL)) Om[i] += dt * expf(lnA - EaR / (T[i] + 273.15f));
+        if (!(flags[i] & F_ABL)) Om[i] += dt * expf(lnA - EaR * __builtin_amdgcn_rcpf(T[i] + 273.15f));
 }
 
 __global__ void k_char(const float *__restrict__ T, float *__restrict__ chi, const uint8_t *__restrict__ flags,
@@ -789,7 +817,7 @@ __global__ void k_char(const float *__restrict__ T, float *__restrict__ chi, con
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint8_t fl = flags[i];
         if ((fl & F_DRY) && !(fl & F_ABL)) {
-            const float rate = expf(lnA - EaR / (T[i] + 273.15f));
+            const float rate = expf(lnA - EaR * __builtin_amdgcn_rcpf(T[i] + 273.15f));
             chi[i] = 1.0f - (1.0f - chi[i]) * expf(-rate * dt);
         }
     }
