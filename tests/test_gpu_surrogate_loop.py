@@ -101,3 +101,43 @@ def test_loop_judges_every_tenth_epoch_and_falls_back():
         assert [i for i, h in enumerate(sur.history) if h["judge_error"] is not None] == idx
         sur.close()
         eng.close()
+
+
+def test_network_gains_over_the_stale_field_in_the_loop(oracle):
+    """What the trained network gains in the coupled loop over the method without it.  Between
+    MC epochs a loop without the network has the field of its last MC epoch (persistence); the
+    surrogate loop has the network's tiled windows, their 8^3-block totals taken from a
+    37 500-photon MC of the current state.  At every judge epoch of a run of the bench workload
+    (256x256x192, judgements every third epoch) both are compared with that epoch's 150k-photon
+    MC over 4^3-block sums of non-ablated voxels (the loop's own metric): the network's field
+    must be closer.  The low-photon run's own distance to the epoch's MC, which the gate uses as
+    the MC-vs-MC noise, is printed beside them.  Measured: network 6.4-11.1 %, stale field
+    25-79 %, the low-photon run alone 8.0-10.5 % (the network's shape inside the 8^3 blocks
+    beats that run's own shape in the first epochs and matches it later)."""
+    from surrogate.coupled import SurrogateCoupledEngine
+    from simulation.coupled_ablation import CoupledAblationEngine
+    from simulation.workloads import layered_stack
+    from oracle import surrogate_oracle as S
+    eng = CoupledAblationEngine(**layered_stack())
+    sur = SurrogateCoupledEngine(eng, WEIGHTS, judge_every=3, tol=10.0, mc_photons_lo=37_500)
+    q_last = None
+    rows = []
+    for _ in range(13):
+        d = sur.step_epoch()
+        if d['optics_source'] != 'mc':
+            continue
+        q_mc = np.array(eng.Q, dtype=np.float64)
+        if d['judge_error'] is not None and q_last is not None:
+            abl = np.array(eng.ablated, dtype=bool)
+            e_stale = S.rel_error(q_mc, q_last[1], abl)[1]
+            d_lo = math.sqrt(d['mc_vs_mc'] ** 2 * (eng.n_photons / 37_500 + 1) / 2)
+            rows.append((d['epoch'], d['epoch'] - q_last[0], d['judge_error'][1], e_stale, d_lo))
+        q_last = (d['epoch'], q_mc)
+    sur.close()
+    eng.close()
+    for r in rows:
+        print(f"epoch {r[0]} (stale field {r[1]} epochs old): network {r[2]:.3f}, stale {r[3]:.3f}, "
+              f"37.5k-photon MC {r[4]:.3f}")
+    assert len(rows) >= 4
+    assert all(r[2] < r[3] for r in rows)
+    assert np.mean([r[2] for r in rows]) < 0.6 * np.mean([r[3] for r in rows])
