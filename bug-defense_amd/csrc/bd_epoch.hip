@@ -11,6 +11,8 @@
 // The host waits once for the diagnostics at the end (callers receive n_new).
 #include "bd_internal.hpp"
 
+#include <atomic>
+#include <utility>
 #include <vector>
 
 #include <cmath>
@@ -302,6 +304,64 @@ static int upload_epoch_params(Ctx *c, int n_substeps, double t_before, int64_t 
 }
 }  // namespace bd
 
+namespace bd {
+static std::atomic<uint64_t> g_graph_seq{0};
+
+// The context's captured epoch graph for (beam, n_substeps), captured again when the beam, the
+// sub-step count, the ping-pong heads, physics / plume values or parameter buffers changed.
+static int ensure_epoch_graph(Ctx *c, const bd_beam *beam, int n_substeps)
+{
+    const bool recapture = !c->graph_exec || c->graph_nsub != n_substeps || !same_beam(c->graph_beam, *beam) ||
+                           c->graph_T != c->T || c->graph_P != c->P || c->graph_H != c->H ||
+                           c->graph_F != c->flags || c->graph_stale;
+    if (!recapture) return BD_OK;
+    // allocate every lazily created buffer and tune the tiles before capture
+    BD_TRY(launch_substeps(c, 0, 1, BD_SUB_PLATEAU, true));
+    if (beam->power > 0.0) BD_TRY(launch_pack_optics(c, beam->add_plume != 0));
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    if (c->graph) {
+        (void)hipGraphDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    c->graph_version = ++g_graph_seq;  // unique over all contexts, so a parent never reuses a stale child
+    // buffers used inside the graph must exist before capture
+    BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
+    BD_TRY(ensure(c, (void **)&c->optics, (size_t)c->nvox * sizeof(float4)));
+    BD_TRY(ensure(c, (void **)&c->hlat, (size_t)c->nvox * 4));
+    BD_TRY(ensure_records(c, beam, REC_MAIN));
+    const int64_t nd = beam->launch_axis == 0 ? c->g.nz : c->g.nx;
+    if (c->lat_n != nd) {
+        if (c->lat) BD_HIP(hipFree(c->lat));
+        c->lat = nullptr;
+        BD_HIP(hipMalloc((void **)&c->lat, sizeof(double) * nd));
+        c->lat_n = nd;
+    }
+    BD_HIP(hipStreamSynchronize(c->stream));
+    c->graph_T = c->T;
+    c->graph_P = c->P;
+    c->graph_H = c->H;
+    c->graph_F = c->flags;
+    BD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_epoch(c, beam, n_substeps);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != BD_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    BD_HIP(e);
+    c->graph = g;
+    BD_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    c->graph_beam = *beam;
+    c->graph_nsub = n_substeps;
+    c->graph_stale = false;
+    return BD_OK;
+}
+}  // namespace bd
+
 extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_before,
                                      int64_t epoch, uint64_t seed_base)
 {
@@ -309,53 +369,7 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
     BD_CHECK(n_substeps >= 1, "bd_epoch_graph: n_substeps must be >= 1");
     Ctx *c = &h->c;
     BD_TRY(prepare_enqueued_epoch(c, n_substeps, "bd_epoch_graph"));
-    const bool recapture = !c->graph_exec || c->graph_nsub != n_substeps || !same_beam(c->graph_beam, *beam) ||
-                           c->graph_T != c->T || c->graph_P != c->P || c->graph_H != c->H ||
-                           c->graph_F != c->flags || c->graph_stale;
-    if (recapture) {
-        // allocate every lazily created buffer and tune the tiles before capture
-        BD_TRY(launch_substeps(c, 0, 1, BD_SUB_PLATEAU, true));
-        if (beam->power > 0.0) BD_TRY(launch_pack_optics(c, beam->add_plume != 0));
-        if (c->graph_exec) {
-            (void)hipGraphExecDestroy(c->graph_exec);
-            c->graph_exec = nullptr;
-        }
-        if (c->graph) {
-            (void)hipGraphDestroy(c->graph);
-            c->graph = nullptr;
-        }
-        // buffers used inside the graph must exist before capture
-        BD_TRY(ensure(c, (void **)&c->AL, (size_t)c->nvox * sizeof(float2)));
-        BD_TRY(ensure(c, (void **)&c->optics, (size_t)c->nvox * sizeof(float4)));
-        BD_TRY(ensure(c, (void **)&c->hlat, (size_t)c->nvox * 4));
-        BD_TRY(ensure_records(c, beam, REC_MAIN));
-        const int64_t nd = beam->launch_axis == 0 ? c->g.nz : c->g.nx;
-        if (c->lat_n != nd) {
-            if (c->lat) BD_HIP(hipFree(c->lat));
-            c->lat = nullptr;
-            BD_HIP(hipMalloc((void **)&c->lat, sizeof(double) * nd));
-            c->lat_n = nd;
-        }
-        BD_HIP(hipStreamSynchronize(c->stream));
-        c->graph_T = c->T;
-        c->graph_P = c->P;
-        c->graph_H = c->H;
-        c->graph_F = c->flags;
-        BD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_epoch(c, beam, n_substeps);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc != BD_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        BD_HIP(e);
-        c->graph = g;
-        BD_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-        c->graph_beam = *beam;
-        c->graph_nsub = n_substeps;
-        c->graph_stale = false;
-    }
+    BD_TRY(ensure_epoch_graph(c, beam, n_substeps));
     // per-epoch scalars -> device (outside the graph), replay; the readback is bd_epoch_graph_finish
     const double dt_epoch = n_substeps * c->phys.dt_sub;
     BD_HIP(hipEventRecord(c->ev2, c->stream));
@@ -365,6 +379,87 @@ extern "C" int bd_epoch_graph_launch(bd_ctx *h, const bd_beam *beam, int n_subst
     c->graph_pending = true;
     c->pending_beam = *beam;
     c->pending_dt_epoch = dt_epoch;
+    return BD_OK;
+}
+
+// ---- one launch for an epoch of many contexts: their captured epoch graphs as independent
+// child nodes of one parent graph, launched once on a library stream of the device.  The parent
+// is rebuilt only when the set of contexts or one of their epoch graphs changed.  Library-wide
+// per device, like the gang transport: batch calls on one device must not overlap in time.
+// Measured on the 24-run campaign: the runtime runs the child graphs one after another (the
+// sequential rate, half that of one stream per run); a graph captured across the contexts'
+// streams with the epochs as parallel branches ran the same way.
+namespace bd {
+namespace {
+struct BatchRes {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<const Ctx *, uint64_t>> key;  // (context, its graph's capture id) per child
+};
+BatchRes g_batch[16];
+}  // namespace
+}  // namespace bd
+
+extern "C" int bd_epoch_batch_launch(bd_ctx *const *hs, int n, const bd_beam *beams, int n_substeps,
+                                     const double *t_before, const int64_t *epochs, const uint64_t *seed_bases)
+{
+    using namespace bd;
+    BD_CHECK(hs && beams && t_before && epochs && seed_bases, "bd_epoch_batch_launch: null argument");
+    BD_CHECK(n >= 1 && n <= 4096, "bd_epoch_batch_launch: n must be in [1, 4096]");
+    BD_CHECK(n_substeps >= 1, "bd_epoch_batch_launch: n_substeps must be >= 1");
+    int dev = 0;
+    BD_HIP(hipGetDevice(&dev));
+    BD_CHECK(dev >= 0 && dev < 16, "bd_epoch_batch_launch: device index out of range");
+    BatchRes &b = g_batch[dev];
+    if (!b.stream) {
+        BD_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+        BD_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    }
+    std::vector<std::pair<const Ctx *, uint64_t>> key;
+    for (int i = 0; i < n; ++i) {
+        BD_CHECK(hs[i], "bd_epoch_batch_launch: null context");
+        for (int j = 0; j < i; ++j) BD_CHECK(hs[j] != hs[i], "bd_epoch_batch_launch: a context appears twice");
+        Ctx *c = &hs[i]->c;
+        BD_CHECK(c->device == dev, "bd_epoch_batch_launch: every context must be on the current device");
+        BD_TRY(prepare_enqueued_epoch(c, n_substeps, "bd_epoch_batch_launch"));
+        BD_TRY(ensure_epoch_graph(c, &beams[i], n_substeps));
+        key.emplace_back(c, c->graph_version);
+    }
+    if (!b.exec || key != b.key) {
+        if (b.exec) (void)hipGraphExecDestroy(b.exec);
+        if (b.graph) (void)hipGraphDestroy(b.graph);
+        b.exec = nullptr;
+        b.graph = nullptr;
+        b.key.clear();
+        BD_HIP(hipGraphCreate(&b.graph, 0));
+        for (int i = 0; i < n; ++i) {
+            hipGraphNode_t node;
+            BD_HIP(hipGraphAddChildGraphNode(&node, b.graph, nullptr, 0, hs[i]->c.graph));
+        }
+        BD_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+        b.key = key;
+    }
+    // each context's per-epoch scalars, queued on its own stream; the batch stream waits for them
+    for (int i = 0; i < n; ++i) {
+        Ctx *c = &hs[i]->c;
+        BD_TRY(upload_epoch_params(c, n_substeps, t_before[i], epochs[i], seed_bases[i]));
+        BD_HIP(hipEventRecord(c->ev_fork, c->stream));
+        BD_HIP(hipStreamWaitEvent(b.stream, c->ev_fork, 0));
+    }
+    for (int i = 0; i < n; ++i) BD_HIP(hipEventRecord(hs[i]->c.ev2, b.stream));
+    BD_HIP(hipGraphLaunch(b.exec, b.stream));
+    BD_HIP(hipEventRecord(b.done, b.stream));
+    // every context's stream continues after the launch (bd_epoch_graph_finish waits on it)
+    for (int i = 0; i < n; ++i) {
+        Ctx *c = &hs[i]->c;
+        BD_HIP(hipStreamWaitEvent(c->stream, b.done, 0));
+        BD_HIP(hipEventRecord(c->ev3, c->stream));
+        c->graph_pending = true;
+        c->pending_beam = beams[i];
+        c->pending_dt_epoch = n_substeps * c->phys.dt_sub;
+    }
     return BD_OK;
 }
 

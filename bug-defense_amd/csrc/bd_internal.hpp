@@ -148,6 +148,7 @@ struct Ctx {
     const uint8_t *graph_F = nullptr;
     bool graph_pending = false;          // launched by bd_epoch_graph_launch, not yet finished
     bool graph_stale = false;            // physics/plume values or parameter buffers changed since capture
+    uint64_t graph_version = 0;          // capture id, unique over all contexts (parents holding the graph rebuild)
     bd_beam pending_beam{};
     double pending_dt_epoch = 0.0;
 };

@@ -145,6 +145,9 @@ _SIGS = {
     "bd_epoch_gang_launch": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Beam), ctypes.c_int,
                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                               ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "bd_epoch_batch_launch": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Beam), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                               ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "bd_reduce": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                   ctypes.c_int),
     "bd_phase_map": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
@@ -619,6 +622,21 @@ def epoch_gang_launch(ctxs, beams, n_substeps, t_before, epochs, seed_bases):
     es = (ctypes.c_int64 * n)(*[int(e) for e in epochs])
     ss = (ctypes.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in seed_bases])
     check(lib().bd_epoch_gang_launch(hs, n, bs, int(n_substeps), ts, es, ss), "bd_epoch_gang_launch")
+
+
+def epoch_batch_launch(ctxs, beams, n_substeps, t_before, epochs, seed_bases):
+    """One epoch of several contexts as a single graph launch (bd_epoch_batch_launch): their
+    captured epoch graphs are child nodes of one parent graph; collect each context with
+    Context.epoch_graph_finish()."""
+    n = len(ctxs)
+    if not (len(beams) == len(t_before) == len(epochs) == len(seed_bases) == n):
+        raise ValueError("epoch_batch_launch: one beam, time, epoch and seed per context")
+    hs = (ctypes.c_void_p * n)(*[c.handle for c in ctxs])
+    bs = (Beam * n)(*beams)
+    ts = (ctypes.c_double * n)(*[float(t) for t in t_before])
+    es = (ctypes.c_int64 * n)(*[int(e) for e in epochs])
+    ss = (ctypes.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in seed_bases])
+    check(lib().bd_epoch_batch_launch(hs, n, bs, int(n_substeps), ts, es, ss), "bd_epoch_batch_launch")
 
 
 def scratch_context(shape, spacing) -> Context:

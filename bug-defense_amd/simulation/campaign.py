@@ -82,13 +82,14 @@ def campaign_runs(seed=0, n_photons=150_000, voxel_mm=0.015, plume_grid=None, gr
 
 
 def run_campaign(runs, epochs, concurrent=True, progress=None, mode=None):
-    """Run every engine for `epochs` epochs, mode "gang", "concurrent" (graph
-    epochs on one stream per run) or "sequential" (one run after another; the
-    default mode follows `concurrent`).  Returns the wall time of the epoch loop,
+    """Run every engine for `epochs` epochs, mode "concurrent" (graph epochs on
+    one stream per run), "batch" (every run's epoch graph a child of one parent
+    graph, one launch per round), "gang" (one transport launch for all runs) or
+    "sequential" (one run after another; the default mode follows `concurrent`).  Returns the wall time of the epoch loop,
     each run's final diagnostics and the voxel-epochs done."""
     from .coupled_ablation import CoupledAblationEngine
     mode = mode or ("concurrent" if concurrent else "sequential")
-    if mode not in ("gang", "concurrent", "sequential"):
+    if mode not in ("gang", "batch", "concurrent", "sequential"):
         raise ValueError(f"unknown campaign mode {mode!r}")
     engines = [CoupledAblationEngine(graph=True, **kw) for kw in runs]
     try:
@@ -99,9 +100,10 @@ def run_campaign(runs, epochs, concurrent=True, progress=None, mode=None):
             e._ctx.sync()
         t0 = time.perf_counter()
         last = [None] * len(engines)
-        if mode == "gang":
+        if mode in ("gang", "batch"):
+            step = CoupledAblationEngine.step_epochs_gang if mode == "gang" else CoupledAblationEngine.step_epochs_batch
             for ep in range(epochs):
-                last = CoupledAblationEngine.step_epochs_gang(engines)
+                last = step(engines)
                 if progress and (ep + 1) % progress == 0:
                     print(f"campaign: epoch {ep + 1}/{epochs}", flush=True)
         elif mode == "concurrent":
@@ -132,7 +134,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--photons", type=int, default=150_000)
-    ap.add_argument("--mode", default="concurrent", choices=("concurrent", "gang", "sequential"))
+    ap.add_argument("--mode", default="concurrent", choices=("concurrent", "batch", "gang", "sequential"))
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
     runs = campaign_runs(seed=a.seed, n_photons=a.photons)

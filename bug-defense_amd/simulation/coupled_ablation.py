@@ -526,6 +526,31 @@ class CoupledAblationEngine:
             out.append(e._finish_epoch(d, e.t + e.dt_epoch, time.perf_counter() - t0))
         return out
 
+    @staticmethod
+    def step_epochs_batch(engines):
+        """One epoch of every engine as a single graph launch (bd_epoch_batch_launch): each engine's
+        captured epoch graph is a child node of one parent graph.  The engines need a
+        graph-capturable configuration and one sub-step count.  Returns each engine's
+        diagnostics, as step_epoch."""
+        if not engines:
+            return []
+        ns = {e.n_substeps for e in engines}
+        if len(ns) != 1:
+            raise ValueError("step_epochs_batch: the engines must share the sub-step count")
+        beams = []
+        for e in engines:
+            beams.append(e._epoch_beam())
+            if not e.graphable():
+                raise ValueError("step_epochs_batch needs graph-capturable configurations")
+        t0 = time.perf_counter()
+        B.epoch_batch_launch([e._ctx for e in engines], beams, engines[0].n_substeps, [e.t for e in engines],
+                             [e.epoch for e in engines], [e.seed for e in engines])
+        out = []
+        for e in engines:
+            d = e._ctx.epoch_graph_finish()
+            out.append(e._finish_epoch(d, e.t + e.dt_epoch, time.perf_counter() - t0))
+        return out
+
     def step_epoch_finish(self) -> dict:
         t_after, t0 = self._pending
         d = self._ctx.epoch_graph_finish()

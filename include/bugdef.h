@@ -283,6 +283,19 @@ int bd_epoch_graph_finish(bd_ctx *ctx, bd_epoch_diag *out);
 int bd_epoch_gang_launch(bd_ctx *const *ctxs, int n, const bd_beam *beams, int n_substeps, const double *t_before,
                          const int64_t *epochs, const uint64_t *seed_bases);
 
+/* One epoch of n contexts (a campaign) as one graph launch: each context's
+ * captured epoch graph (as bd_epoch_graph_launch captures it) is an
+ * independent child node of one parent graph, launched once on a library
+ * stream of the current device after every context's parameter upload; each
+ * context's stream then continues after it.  The parent graph is rebuilt only
+ * when the contexts or one of their epoch graphs changed.  Same per-context
+ * inputs as bd_epoch_graph_launch (arrays of n, contexts on the current
+ * device); collect each context with bd_epoch_graph_finish.  Batch calls on
+ * one device must not overlap in time.  No reference counterpart: the
+ * reference runs a campaign one run after another (run_multispectral_matrix.py). */
+int bd_epoch_batch_launch(bd_ctx *const *ctxs, int n, const bd_beam *beams, int n_substeps, const double *t_before,
+                          const int64_t *epochs, const uint64_t *seed_bases);
+
 /* Reductions (FP64): op 0 sum, 1 max, 2 min over a field; optional u8 mask
  * field (-1 for none) restricts the reduction to mask != 0. */
 int bd_reduce(bd_ctx *ctx, int field, int op, int mask_field, double *out);

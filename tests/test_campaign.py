@@ -69,3 +69,39 @@ def test_gang_transport_epochs_match_sequential_runs():
     for a, b in zip(s1, s2):
         assert a['absorbed_frac'] == pytest.approx(b['absorbed_frac'], rel=1e-2)
         assert abs(a['removed_total'] - b['removed_total']) <= max(2, 0.01 * b['removed_total'])
+
+
+@pytest.mark.gpu
+def test_batch_graph_epochs_match_sequential_runs():
+    """Every run's captured epoch graph as a child of one parent graph, one launch per round:
+    the same photons and the same sub-steps as the runs one at a time (FP64 scalar tallies equal
+    to round-off in the first epoch, the state after several epochs within the float-atomic
+    spread); the parent is rebuilt when a run's graph is captured again or the run set changes."""
+    from simulation.campaign import run_campaign
+    from simulation.coupled_ablation import CoupledAblationEngine
+    runs = campaign_runs(n_photons=20_000, grids=((48, 48, 64), (64, 48, 72), (40, 56, 64)),
+                         wavelengths=(532.0, 1470.0), powers=(40.0,))
+    ba = [CoupledAblationEngine(graph=True, **kw) for kw in runs]
+    sq = [CoupledAblationEngine(graph=True, **kw) for kw in runs]
+    for _ in range(2):
+        d_b = CoupledAblationEngine.step_epochs_batch(ba)
+        d_s = [e.step_epoch() for e in sq]
+        for a, b in zip(d_b, d_s):
+            assert a['absorbed_frac'] == pytest.approx(b['absorbed_frac'], rel=1e-6)
+            assert a['ledger_J']['reflected'] == pytest.approx(b['ledger_J']['reflected'], rel=1e-6, abs=1e-15)
+            assert a['t_optical_s'] > 0.0 and a['t_thermal_s'] > 0.0
+    # a new run in place of the first one: a context the parent graph has never held
+    ba[0].close()
+    ba[0] = CoupledAblationEngine(graph=True, **runs[0])
+    sq[0].close()
+    sq[0] = CoupledAblationEngine(graph=True, **runs[0])
+    d_b = CoupledAblationEngine.step_epochs_batch(ba)
+    d_s = [e.step_epoch() for e in sq]
+    assert d_b[0]['absorbed_frac'] == pytest.approx(d_s[0]['absorbed_frac'], rel=1e-9)
+    for e in ba + sq:
+        e.close()
+    w1, s1, _ = run_campaign(runs, 6, mode="batch")
+    w2, s2, _ = run_campaign(runs, 6, mode="sequential")
+    for a, b in zip(s1, s2):
+        assert a['absorbed_frac'] == pytest.approx(b['absorbed_frac'], rel=1e-2)
+        assert abs(a['removed_total'] - b['removed_total']) <= max(2, 0.01 * b['removed_total'])
