@@ -627,6 +627,18 @@ int bd_mc_run(bd_ctx *h, const bd_beam *b, bd_mc_result *out)
     return BD_OK;
 }
 
+int bd_mc_launch(bd_ctx *h, const bd_beam *b)
+{
+    BD_CHECK(h && b, "bd_mc_launch: null argument");
+    Ctx *c = &h->c;
+    BD_CHECK(b->n_photons > 0, "bd_mc_launch: n_photons must be positive");
+    BD_CHECK(b->beam_profile >= 0 && b->beam_profile <= 3, "bd_mc_launch: unknown beam profile");
+    BD_CHECK(b->launch_axis == 0 || b->launch_axis == 1, "bd_mc_launch: launch_axis must be 0 (z) or 1 (x)");
+    BD_CHECK(b->max_events > 0, "bd_mc_launch: max_events must be positive");
+    BD_CHECK(c->mu_a && c->mu_s && c->gg && c->nn, "bd_mc_launch: optics (mu_a, mu_s, g, n) not uploaded");
+    return launch_mc_enqueue(c, b, nullptr, 1, REC_MAIN);
+}
+
 int bd_mc_lateral(bd_ctx *h, double *host, int64_t n_depth)
 {
     BD_CHECK(h && host, "bd_mc_lateral: null argument");

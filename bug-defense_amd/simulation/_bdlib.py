@@ -121,6 +121,7 @@ _SIGS = {
     "bd_char": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "bd_plume_substeps": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "bd_mc_run": ([ctypes.c_void_p, ctypes.POINTER(Beam), ctypes.POINTER(MCResult)], ctypes.c_int),
+    "bd_mc_launch": ([ctypes.c_void_p, ctypes.POINTER(Beam)], ctypes.c_int),
     "bd_mc_lateral": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "bd_mc_run_noalloc": ([ctypes.POINTER(Grid), ctypes.POINTER(Beam), ctypes.c_void_p, ctypes.c_void_p,
                            ctypes.c_void_p, ctypes.POINTER(MCResult), ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
@@ -427,6 +428,10 @@ class Context:
         r = MCResult()
         check(lib().bd_mc_run(self.handle, ctypes.byref(beam), ctypes.byref(r)), "bd_mc_run")
         return r
+
+    def mc_launch(self, beam: Beam):
+        """mc_run queued on the context's stream without waiting (bd_mc_launch)."""
+        check(lib().bd_mc_launch(self.handle, ctypes.byref(beam)), "bd_mc_launch")
 
     def mc_lateral(self, n_depth):
         out = np.empty(int(n_depth), dtype=np.float64)

@@ -34,9 +34,11 @@ E d_s^2 = e^2 + s^2 and E d_mm^2 = 2 s^2 (e: the prediction's distance from
 the expected field, s: one run's noise), this asks e^2 <= s^2 + tol^2, i.e.
 the prediction may be off the expected field by what one MC run is, plus tol.
 
-Coarse amplitudes (``mc_photons_lo`` > 0): each prediction first runs a
+Coarse amplitudes (``mc_photons_lo`` > 0): each prediction also runs a
 low-photon MC of the current state (photon ids disjoint from the epoch's own
-stream) and the composed field takes its totals over blocks of
+stream), queued on the engine's stream beside the network's forward pass on
+its own stream (the features do not use it), and the composed field takes its
+totals over blocks of
 ``correct_block``^3 voxels from that run (bd_sur_correct); the network supplies
 the shape inside each block, where the low-photon run is noisiest.  That run's
 absorbed power also replaces the kept field outside the windows.
@@ -137,15 +139,26 @@ class SurrogateCoupledEngine:
         mp = self.net.mc_photons
         lo = mp or self.mc_lo
         t0 = self._mark("windows", t0)
-        if lo:
-            ctx.mc_run(noisy_beam(beam, lo))
+
+        def low_photon_mc(wait):
+            (ctx.mc_run if wait else ctx.mc_launch)(noisy_beam(beam, lo))
             ctx.sur_keep_stale()
             if self.mc_lo:
                 ctx.sur_keep_aux()
+
+        if lo and mp:
+            # a refining network's features hold the low-photon run's fluence: that run first
+            low_photon_mc(True)
             t0 = self._mark("mc_lo", t0)
         ctx.sur_features_n(beam, ws, self.x, mp)
         ctx.sync()
         t0 = self._mark("features", t0)
+        if lo and not mp:
+            # the features do not need the low-photon run: it is queued on the engine's stream and
+            # runs beside the forward pass on the network's stream; the composition below follows
+            # it on the engine's stream
+            low_photon_mc(False)
+            t0 = self._mark("mc_lo", t0)
         self.net.forward(self.x.view(0, (nb, T, T, T, CIN)), self.y.view(0, (nb, T, T, T)))
         t0 = self._mark("network", t0)
         ctx.sur_apply_n(beam, ws, self.y, into_q, self.scale)

@@ -192,6 +192,10 @@ int bd_plume_substeps(bd_ctx *ctx, int n, double dt, double decay);   /* plume.p
  * (voxel_mc_metal.py:388-586, voxel_mc_3d.py:340-422).  Tallies A and L are
  * zeroed first; Q is set to A * power / (n_photons * voxel volume). */
 int bd_mc_run(bd_ctx *ctx, const bd_beam *beam, bd_mc_result *out);
+/* bd_mc_run queued on the context's stream without waiting for it; the tallies
+ * and Q are ready for the context's later calls.  No reference counterpart: the
+ * surrogate loop runs its low-photon transport beside the network's forward pass. */
+int bd_mc_launch(bd_ctx *ctx, const bd_beam *beam);
 /* Lateral escape per depth index (voxel_mc_3d.py:697), n_depth values. */
 int bd_mc_lateral(bd_ctx *ctx, double *host, int64_t n_depth);
 
