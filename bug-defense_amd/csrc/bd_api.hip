@@ -561,11 +561,8 @@ int bd_substeps(bd_ctx *h, int n_substeps, int is_active, int flags)
     if (flags & BD_SUB_PEAK) BD_TRY(ensure(c, (void **)&c->Tpeak, (size_t)c->nvox * 4));
     if (n_substeps == 0) return BD_OK;
     BD_TRY(launch_substeps(c, n_substeps, is_active, flags, false));
-    if ((flags & BD_SUB_PLUME) && c->plume.enabled && c->plume.model == 0) {
-        const double tl = c->plume.tau_linger;
-        const double decay = (tl > 0.0 && std::isfinite(tl)) ? std::exp(-c->phys.dt_sub / tl) : 1.0;
-        BD_TRY(launch_plume_substeps(c, n_substeps, c->phys.dt_sub, decay));
-    }
+    if ((flags & BD_SUB_PLUME) && c->plume.enabled && c->plume.model == 0)
+        BD_TRY(launch_plume_substeps(c, n_substeps, c->phys.dt_sub, plume_decay(c, c->phys.dt_sub)));
     BD_HIP(hipStreamSynchronize(c->stream));
     return BD_OK;
 }

@@ -90,11 +90,7 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
     BD_HIP(hipEventRecord(c->ev4, c->stream));
     BD_TRY(launch_substeps(c, n_substeps, 1, flags, true));
     BD_HIP(hipEventRecord(c->ev5, c->stream));
-    if (diff_plume) {
-        const double tl = c->plume.tau_linger;
-        const double decay = (tl > 0.0 && std::isfinite(tl)) ? std::exp(-p.dt_sub / tl) : 1.0;
-        BD_TRY(launch_plume_substeps(c, n_substeps, p.dt_sub, decay));
-    }
+    if (diff_plume) BD_TRY(launch_plume_substeps(c, n_substeps, p.dt_sub, plume_decay(c, p.dt_sub)));
     BD_HIP(hipEventRecord(c->ev8, c->stream));
 
     // ---- 3. phase update + removal
@@ -220,11 +216,7 @@ static int enqueue_epoch_tail(Ctx *c, int n_substeps)
     BD_HIP(hipEventRecordWithFlags(c->gev[3], c->stream, hipEventRecordExternal));
     BD_TRY(launch_substeps(c, n_substeps, 1, flags, true));
     BD_HIP(hipEventRecordWithFlags(c->gev[4], c->stream, hipEventRecordExternal));
-    if (diff_plume) {
-        const double tl = c->plume.tau_linger;
-        const double decay = (tl > 0.0 && std::isfinite(tl)) ? std::exp(-p.dt_sub / tl) : 1.0;
-        BD_TRY(launch_plume_substeps(c, n_substeps, p.dt_sub, decay));
-    }
+    if (diff_plume) BD_TRY(launch_plume_substeps(c, n_substeps, p.dt_sub, plume_decay(c, p.dt_sub)));
     BD_HIP(hipEventRecordWithFlags(c->gev[5], c->stream, hipEventRecordExternal));
     BD_TRY(launch_phase_enqueue(c, 0.0, c->ep));
     if (plume) {

@@ -186,6 +186,8 @@ int ensure_records(Ctx *c, const bd_beam *b, int rec_mode);  // record buffers, 
 // launch helpers implemented in the kernel translation units
 int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger);
 int launch_plume_substeps(Ctx *c, int n, double dt, double decay);
+// exp(-dt / tau_linger), 1 without lingering decay
+double plume_decay(const Ctx *c, double dt);
 int launch_plateau(Ctx *c);
 int launch_arrhenius(Ctx *c, double A, double EaR, double dt);
 int launch_char(Ctx *c, double A, double EaR, double dt);

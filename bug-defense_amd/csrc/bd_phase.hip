@@ -48,90 +48,143 @@ struct PhaseArgs {
 __device__ __forceinline__ float pf(const PParam &p, int64_t i) { return p.per_voxel ? p.f[i] : p.f[0]; }
 __device__ __forceinline__ double pd(const PParam &p, int64_t i) { return p.per_voxel ? p.d[i] : p.d[0]; }
 
+// One intact cell i (flags fl without F_ABL) with its temperature T, banked
+// enthalpy h (0 unless F_HPOS) and latent heat hl_raw already loaded: the phase
+// update and, when the cell is marked, its conversion to void.  Returns the new
+// flags; rem says whether the cell was marked, e_abl gains its removed energy.
+__device__ __forceinline__ uint8_t phase_cell(const PhaseArgs &a, int64_t i, uint8_t fl, float T, float h,
+                                              float hl_raw, double t_now, bool &rem, double &e_abl)
+{
+    rem = false;
+    if (a.mode == 0) {
+        rem = ((double)h >= (double)hl_raw) && ((double)T >= a.t_abl);
+    } else {
+        bool is_dry = (fl & F_DRY) != 0;
+        if (!is_dry && h > 0.f) {
+            const float hl = fmaxf(hl_raw, 1e-30f);
+            const float frac = fminf(fmaxf(h / hl, 0.f), 1.f);
+            const float dmu = pf(a.ds[0], i), dk = pf(a.ds[4], i), drc = pf(a.ds[5], i);
+            const float wmu = a.wet_mu_a[i], wk = a.wet_k[i], wrc = a.wet_rc[i];
+            a.mu_a[i] = wmu + (dmu - wmu) * frac;
+            a.k[i] = wk + (dk - wk) * frac;
+            a.rc[i] = wrc + (drc - wrc) * frac;
+            if ((double)h >= (double)hl_raw) {
+                is_dry = true;
+                fl = (uint8_t)((fl | F_DRY) & ~F_HPOS);
+                a.H[i] = 0.f;
+                h = 0.f;
+                a.mu_a[i] = dmu;
+                a.mu_s[i] = pf(a.ds[1], i);
+                a.g[i] = pf(a.ds[2], i);
+                a.n[i] = pf(a.ds[3], i);
+                a.k[i] = dk;
+                a.rc[i] = drc;
+            }
+        }
+        if (is_dry) {
+            if (a.has_cs) {
+                const float dmu = pf(a.ds[0], i), dk = pf(a.ds[4], i), drc = pf(a.ds[5], i);
+                const float cv = a.chi[i];
+                if (a.has_cs_mu_a) a.mu_a[i] = dmu + cv * (pf(a.cs[0], i) - dmu);
+                if (a.has_cs_mu_s) {
+                    const float dms = pf(a.ds[1], i);
+                    a.mu_s[i] = dms + cv * (pf(a.cs[1], i) - dms);
+                }
+                float kk;
+                if (a.has_cs_k) {
+                    kk = dk + cv * (pf(a.cs[2], i) - dk);
+                    a.k[i] = kk;
+                } else {
+                    kk = a.k[i];
+                }
+                float rf = a.has_cs_rc ? drc + cv * (pf(a.cs[3], i) - drc) : a.rc[i];
+                const float mn = (float)((double)kk / a.alpha_max);
+                if (rf < mn) rf = mn;
+                a.rc[i] = rf;
+                rem = ((double)T >= pd(a.cs[4], i)) || ((double)cv >= a.cs_chi_rem);
+            } else {
+                rem = (double)T >= pd(a.ds[6], i);
+            }
+        }
+    }
+    if (rem && !a.no_removal) {
+        // _remove (coupled_ablation.py:632-643) on this cell
+        e_abl += a.vol * ((double)a.rc[i] * ((double)T - a.t_amb) + (double)h);
+        fl = (uint8_t)((fl | F_ABL) & ~(F_DRY | F_HPOS));
+        a.chi[i] = 0.f;
+        if (a.abl_time) a.abl_time[i] = t_now;
+        a.mu_a[i] = 0.f;
+        a.mu_s[i] = 0.f;
+        a.g[i] = 0.f;
+        a.n[i] = 1.f;
+        a.k[i] = a.k_void;
+        a.rc[i] = a.rc_void;
+        a.T[i] = a.t_void;
+        a.H[i] = 0.f;
+    }
+    return fl;
+}
+
+// Four consecutive cells per thread and iteration: their flags, T and latent heat
+// come in one vector load each, issued together before any branch (the banked
+// enthalpy only where a cell has some), so a quad waits on memory once; the cells are then updated in
+// order by phase_cell, and the new flags and removal marks go out as one 4-byte
+// store each.  Ablated cells keep their flags and get a zero mark.  The last
+// nvox % 4 cells are taken one at a time by the first threads of block 0.
 __global__ __launch_bounds__(256) void k_phase(PhaseArgs a)
 {
     unsigned cnt = 0;
     double e_abl = 0.0;
     const double t_now = a.ep ? a.ep->t + a.ep->dt_epoch : a.t_now;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.nvox; i += (int64_t)gridDim.x * blockDim.x) {
-        uint8_t fl = a.flags[i];
-        if (fl & F_ABL) {
-            a.to_remove[i] = 0;
+    const int64_t nq = a.nvox >> 2;
+    const uint32_t *fl4 = (const uint32_t *)a.flags;
+    const float4 *T4 = (const float4 *)a.T;
+    const float4 *hl4 = (const float4 *)a.hlat;
+    const float4 *H4 = (const float4 *)a.H;
+    const uint32_t hpos4 = (uint32_t)F_HPOS * 0x01010101u, abl4 = (uint32_t)F_ABL * 0x01010101u;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nq; q += (int64_t)gridDim.x * blockDim.x) {
+        // the three loads are issued together, ahead of the branch on the flags (the empty
+        // asm takes their results, so the compiler cannot sink the T and latent-heat loads
+        // behind that branch)
+        uint32_t fw = fl4[q];
+        float4 tq = T4[q];
+        float4 lq = hl4[q];
+        asm volatile("" : "+v"(fw), "+v"(tq.x), "+v"(lq.x));
+        if ((fw & abl4) == abl4) {  // four ablated cells
+            ((uint32_t *)a.to_remove)[q] = 0u;
             continue;
         }
-        const float T = a.T[i];
-        float h = (fl & F_HPOS) ? a.H[i] : 0.f;
+        float4 hq = make_float4(0.f, 0.f, 0.f, 0.f);  // banked enthalpy, where a cell has some
+        if (fw & hpos4) hq = H4[q];
+        const float tv[4] = {tq.x, tq.y, tq.z, tq.w};
+        const float lv[4] = {lq.x, lq.y, lq.z, lq.w};
+        const float hv[4] = {hq.x, hq.y, hq.z, hq.w};
+        uint32_t fo = 0u, ro = 0u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint8_t fl = (uint8_t)(fw >> (8 * e));
+            bool rem = false;
+            if (!(fl & F_ABL)) {
+                fl = phase_cell(a, 4 * q + e, fl, tv[e], (fl & F_HPOS) ? hv[e] : 0.f, lv[e], t_now, rem, e_abl);
+                if (rem) ++cnt;
+            }
+            fo |= (uint32_t)fl << (8 * e);
+            ro |= (rem ? 1u : 0u) << (8 * e);
+        }
+        ((uint32_t *)a.flags)[q] = fo;
+        ((uint32_t *)a.to_remove)[q] = ro;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (a.nvox & 3)) {
+        const int64_t i = 4 * nq + threadIdx.x;
+        const uint8_t fl = a.flags[i];
         bool rem = false;
-        if (a.mode == 0) {
-            rem = ((double)h >= (double)a.hlat[i]) && ((double)T >= a.t_abl);
-        } else {
-            bool is_dry = (fl & F_DRY) != 0;
-            const float hl_raw = a.hlat[i];
-            if (!is_dry && h > 0.f) {
-                const float hl = fmaxf(hl_raw, 1e-30f);
-                const float frac = fminf(fmaxf(h / hl, 0.f), 1.f);
-                const float dmu = pf(a.ds[0], i), dk = pf(a.ds[4], i), drc = pf(a.ds[5], i);
-                const float wmu = a.wet_mu_a[i], wk = a.wet_k[i], wrc = a.wet_rc[i];
-                a.mu_a[i] = wmu + (dmu - wmu) * frac;
-                a.k[i] = wk + (dk - wk) * frac;
-                a.rc[i] = wrc + (drc - wrc) * frac;
-                if ((double)h >= (double)hl_raw) {
-                    is_dry = true;
-                    fl = (uint8_t)((fl | F_DRY) & ~F_HPOS);
-                    a.H[i] = 0.f;
-                    h = 0.f;
-                    a.mu_a[i] = dmu;
-                    a.mu_s[i] = pf(a.ds[1], i);
-                    a.g[i] = pf(a.ds[2], i);
-                    a.n[i] = pf(a.ds[3], i);
-                    a.k[i] = dk;
-                    a.rc[i] = drc;
-                }
-            }
-            if (is_dry) {
-                if (a.has_cs) {
-                    const float dmu = pf(a.ds[0], i), dk = pf(a.ds[4], i), drc = pf(a.ds[5], i);
-                    const float cv = a.chi[i];
-                    if (a.has_cs_mu_a) a.mu_a[i] = dmu + cv * (pf(a.cs[0], i) - dmu);
-                    if (a.has_cs_mu_s) {
-                        const float dms = pf(a.ds[1], i);
-                        a.mu_s[i] = dms + cv * (pf(a.cs[1], i) - dms);
-                    }
-                    float kk;
-                    if (a.has_cs_k) {
-                        kk = dk + cv * (pf(a.cs[2], i) - dk);
-                        a.k[i] = kk;
-                    } else {
-                        kk = a.k[i];
-                    }
-                    float rf = a.has_cs_rc ? drc + cv * (pf(a.cs[3], i) - drc) : a.rc[i];
-                    const float mn = (float)((double)kk / a.alpha_max);
-                    if (rf < mn) rf = mn;
-                    a.rc[i] = rf;
-                    rem = ((double)T >= pd(a.cs[4], i)) || ((double)cv >= a.cs_chi_rem);
-                } else {
-                    rem = (double)T >= pd(a.ds[6], i);
-                }
-            }
+        if (!(fl & F_ABL)) {
+            const uint8_t f2 = phase_cell(a, i, fl, a.T[i], (fl & F_HPOS) ? a.H[i] : 0.f, a.hlat[i], t_now, rem,
+                                          e_abl);
+            if (rem) ++cnt;
+            a.flags[i] = f2;
         }
-        if (rem && !a.no_removal) {
-            // _remove (coupled_ablation.py:632-643) on this cell
-            e_abl += a.vol * ((double)a.rc[i] * ((double)T - a.t_amb) + (double)h);
-            fl = (uint8_t)((fl | F_ABL) & ~(F_DRY | F_HPOS));
-            a.chi[i] = 0.f;
-            if (a.abl_time) a.abl_time[i] = t_now;
-            a.mu_a[i] = 0.f;
-            a.mu_s[i] = 0.f;
-            a.g[i] = 0.f;
-            a.n[i] = 1.f;
-            a.k[i] = a.k_void;
-            a.rc[i] = a.rc_void;
-            a.T[i] = a.t_void;
-            a.H[i] = 0.f;
-        }
-        if (rem) ++cnt;
         a.to_remove[i] = rem ? 1 : 0;
-        a.flags[i] = fl;
     }
     // wave-level count + energy, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -240,7 +293,12 @@ int launch_phase_enqueue(Ctx *c, double t_now, const EpochParams *ep)
     a.nvox = c->nvox;
     BD_TRY(launch_fill(c, c->counter + 1, K_F64, 1, 0.0));
     BD_TRY(launch_fill(c, c->red + 9, K_F64, 1, 0.0));
-    const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
+    // the quad loads of k_phase need 16-byte aligned fields (hipMalloc gives more)
+    BD_CHECK(((uintptr_t)c->T | (uintptr_t)c->H | (uintptr_t)c->hlat | (uintptr_t)c->flags |
+              (uintptr_t)c->to_remove) % 16 == 0,
+             "bd_phase_update: state fields are not 16-byte aligned");
+    const int64_t nq = c->nvox / 4;
+    const int nb = nq == 0 ? 1 : (blocks_for(nq, 256) > 8192 ? 8192 : (int)blocks_for(nq, 256));
     hipLaunchKernelGGL(k_phase, dim3(nb), dim3(256), 0, c->stream, a);
     BD_HIP(hipGetLastError());
     if (!p.no_removal && p.air_gap_k0 > 0 && p.air_gap_k1 > p.air_gap_k0) {

@@ -23,39 +23,55 @@ __device__ __forceinline__ int cmap(int i, int n_from, int n_to)
     return (int)(((int64_t)(2 * i + 1) * n_to) / (2 * (int64_t)n_from));
 }
 
-// plume.py:18-80 (FP32 arithmetic)
-__global__ __launch_bounds__(256) void k_plume_diffuse(const float *__restrict__ C, float *__restrict__ Cn,
-                                                        const uint8_t *__restrict__ vm, int Nx, int Ny, int Nz,
-                                                        float idx2, float idy2, float idz2, float inv_dz, float dt,
-                                                        float Dxy, float Dz, float vz, float decay)
+// One explicit step of the plume diffusion (plume.py:18-80, FP32 arithmetic)
+// on the plume grid (Nx, Ny, Nz), z fastest: C in, Cn out, vm the void mask.
+struct PlumeStep {
+    const float *__restrict__ C;
+    float *__restrict__ Cn;
+    const uint8_t *__restrict__ vm;
+    int Nx, Ny, Nz;
+    float idx2, idy2, idz2, inv_dz, dt, Dxy, Dz, vz, decay;
+};
+
+// The step on the z row (i, j), its cells taken 64 at a time by the lanes of one
+// wave, so every neighbour load is one coalesced row access.  Indices are 32-bit
+// (plume grids have fewer than 2^31 cells; the launcher checks).
+__device__ __forceinline__ void plume_row(const PlumeStep &p, int i, int j, int lane)
 {
-    const int64_t N = (int64_t)Nx * Ny * Nz;
-    const int64_t pl = (int64_t)Ny * Nz;
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
-        if (!vm[c]) {
-            Cn[c] = 0.f;
+    const int Ny = p.Ny, Nz = p.Nz;
+    const int pl = Ny * Nz;
+    const int row = (i * Ny + j) * Nz;
+    const bool edge_row = i == 0 || i == p.Nx - 1 || j == 0 || j == Ny - 1;
+    for (int k = lane; k < Nz; k += 64) {
+        const int c = row + k;
+        if (!p.vm[c] || edge_row || k == 0 || k == Nz - 1) {
+            p.Cn[c] = 0.f;
             continue;
         }
-        const int k = (int)(c % Nz);
-        const int64_t r = c / Nz;
-        const int j = (int)(r % Ny), i = (int)(r / Ny);
-        if (i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1 || k == 0 || k == Nz - 1) {
-            Cn[c] = 0.f;
-            continue;
-        }
-        const float Cc = C[c];
-        const float Cw = vm[c - pl] ? C[c - pl] : Cc;
-        const float Ce = vm[c + pl] ? C[c + pl] : Cc;
-        const float Cs = vm[c - Nz] ? C[c - Nz] : Cc;
-        const float Cq = vm[c + Nz] ? C[c + Nz] : Cc;
-        const float Cb = vm[c - 1] ? C[c - 1] : Cc;
-        const float Ct = vm[c + 1] ? C[c + 1] : Cc;
-        const float diff = Dxy * ((Cw - 2.f * Cc + Ce) * idx2 + (Cs - 2.f * Cc + Cq) * idy2) +
-                           Dz * ((Cb - 2.f * Cc + Ct) * idz2);
-        const float adv = vz > 0.f ? -vz * (Cc - Ct) * inv_dz : 0.f;
-        const float v = (Cc + dt * (diff + adv)) * decay;
-        Cn[c] = fmaxf(v, 0.f);
+        const float Cc = p.C[c];
+        const float Cw = p.vm[c - pl] ? p.C[c - pl] : Cc;
+        const float Ce = p.vm[c + pl] ? p.C[c + pl] : Cc;
+        const float Cs = p.vm[c - Nz] ? p.C[c - Nz] : Cc;
+        const float Cq = p.vm[c + Nz] ? p.C[c + Nz] : Cc;
+        const float Cb = p.vm[c - 1] ? p.C[c - 1] : Cc;
+        const float Ct = p.vm[c + 1] ? p.C[c + 1] : Cc;
+        const float diff = p.Dxy * ((Cw - 2.f * Cc + Ce) * p.idx2 + (Cs - 2.f * Cc + Cq) * p.idy2) +
+                           p.Dz * ((Cb - 2.f * Cc + Ct) * p.idz2);
+        const float adv = p.vz > 0.f ? -p.vz * (Cc - Ct) * p.inv_dz : 0.f;
+        const float v = (Cc + p.dt * (diff + adv)) * p.decay;
+        p.Cn[c] = fmaxf(v, 0.f);
     }
+}
+
+// plume.py:18-80: one diffusion step, PD_ROWS z rows of the plume grid per
+// block (one wave per row).  The row comes from the launch shape (blockIdx.y =
+// x, blockIdx.x = group of y rows), so no lane divides a linear index.
+constexpr int PD_ROWS = 4;
+
+__global__ __launch_bounds__(64 * PD_ROWS) void k_plume_diffuse(PlumeStep p)
+{
+    const int j = blockIdx.x * PD_ROWS + threadIdx.y;
+    if (j < p.Ny) plume_row(p, blockIdx.y, j, threadIdx.x);
 }
 
 static int ew_blocks(int64_t n) { return blocks_for(n, 256) > 8192 ? 8192 : blocks_for(n, 256); }
@@ -67,9 +83,10 @@ static void plume_spacing(Ctx *c, double &dx, double &dy, double &dz)
     dz = c->g.dz * c->g.nz / c->plume.nz;
 }
 
-int launch_plume_substeps(Ctx *c, int n, double dt, double decay)
+static int plume_step_args(Ctx *c, double dt, double decay, PlumeStep &ps)
 {
     const bd_plume_params &p = c->plume;
+    BD_CHECK(c->pvox < (int64_t(1) << 31) && p.nx <= 65535, "plume grid too large for the diffusion kernel");
     double dx, dy, dz;
     plume_spacing(c, dx, dy, dz);
     if (p.dt_clamp) {
@@ -78,15 +95,47 @@ int launch_plume_substeps(Ctx *c, int n, double dt, double decay)
         const double d_rate = p.D_xy * (1.0 / (dx * dx) + 1.0 / (dy * dy)) + p.D_z / (dz * dz);
         if (d_rate > 1e-6) dt = std::min(dt, 0.45 / (2.0 * d_rate));
     }
+    ps = PlumeStep{};
+    ps.vm = c->pvoid;
+    ps.Nx = p.nx;
+    ps.Ny = p.ny;
+    ps.Nz = p.nz;
+    ps.idx2 = (float)(1.0 / (dx * dx));
+    ps.idy2 = (float)(1.0 / (dy * dy));
+    ps.idz2 = (float)(1.0 / (dz * dz));
+    ps.inv_dz = (float)(1.0 / dz);
+    ps.dt = (float)dt;
+    ps.Dxy = (float)p.D_xy;
+    ps.Dz = (float)p.D_z;
+    ps.vz = (float)p.v_drift;
+    ps.decay = (float)decay;
+    return BD_OK;
+}
+
+static int launch_plume_steps(Ctx *c, int n, PlumeStep ps)
+{
+    const dim3 grid((unsigned)((ps.Ny + PD_ROWS - 1) / PD_ROWS), (unsigned)ps.Nx);
     for (int s = 0; s < n; ++s) {
-        hipLaunchKernelGGL(k_plume_diffuse, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->P2, c->pvoid,
-                           p.nx, p.ny, p.nz, (float)(1.0 / (dx * dx)), (float)(1.0 / (dy * dy)),
-                           (float)(1.0 / (dz * dz)), (float)(1.0 / dz), (float)dt, (float)p.D_xy, (float)p.D_z,
-                           (float)p.v_drift, (float)decay);
+        ps.C = c->P;
+        ps.Cn = c->P2;
+        hipLaunchKernelGGL(k_plume_diffuse, grid, dim3(64, PD_ROWS), 0, c->stream, ps);
         BD_HIP(hipGetLastError());
         std::swap(c->P, c->P2);
     }
     return BD_OK;
+}
+
+double plume_decay(const Ctx *c, double dt)
+{
+    const double tl = c->plume.tau_linger;
+    return (tl > 0.0 && std::isfinite(tl)) ? std::exp(-dt / tl) : 1.0;
+}
+
+int launch_plume_substeps(Ctx *c, int n, double dt, double decay)
+{
+    PlumeStep ps;
+    BD_TRY(plume_step_args(c, dt, decay, ps));
+    return launch_plume_steps(c, n, ps);
 }
 
 // any(ablated[:, :, k0-1])  (coupled_ablation.py:624)
@@ -100,18 +149,23 @@ __global__ void k_gap_any(const uint8_t *__restrict__ flags, int Nx, int Ny, int
     if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(out, 1ull);
 }
 
-__global__ void k_plume_void(const uint8_t *__restrict__ flags, const unsigned long long *__restrict__ breach,
-                             uint8_t *__restrict__ vm, int Nx, int Ny, int Nz, int Px, int Py, int Pz, int k0, int k1)
+// The plume void mask, one z row of plume cells per wave (blockIdx.y = I, blockIdx.x =
+// group of PD_ROWS J rows): the thermal cell under each plume cell's centre is void,
+// or the cell is in the air gap once the gap is breached (coupled_ablation.py:588,624).
+__global__ __launch_bounds__(64 * PD_ROWS) void k_plume_void(const uint8_t *__restrict__ flags,
+                                                             const unsigned long long *__restrict__ breach,
+                                                             uint8_t *__restrict__ vm, int Nx, int Ny, int Nz, int Px,
+                                                             int Py, int Pz, int k0, int k1)
 {
-    const int64_t N = (int64_t)Px * Py * Pz;
+    const int I = blockIdx.y, J = blockIdx.x * PD_ROWS + threadIdx.y;
+    if (J >= Py) return;
     const bool br = breach && *breach;
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
-        const int K = (int)(c % Pz);
-        const int64_t r = c / Pz;
-        const int J = (int)(r % Py), I = (int)(r / Py);
-        const int i = cmap(I, Px, Nx), j = cmap(J, Py, Ny), k = cmap(K, Pz, Nz);
-        const bool v = (flags[((int64_t)i * Ny + j) * Nz + k] & F_ABL) || (br && k >= k0 && k < k1);
-        vm[c] = v ? 1 : 0;
+    const int64_t col = ((int64_t)cmap(I, Px, Nx) * Ny + cmap(J, Py, Ny)) * Nz;
+    uint8_t *out = vm + ((int64_t)I * Py + J) * Pz;
+    for (int K = threadIdx.x; K < Pz; K += 64) {
+        const int k = cmap(K, Pz, Nz);
+        const bool v = (flags[col + k] & F_ABL) || (br && k >= k0 && k < k1);
+        out[K] = v ? 1 : 0;
     }
 }
 
@@ -128,29 +182,48 @@ int launch_plume_void(Ctx *c)
                            c->stream, c->flags, c->g.nx, c->g.ny, c->g.nz, ph.air_gap_k0 - 1, breach);
         BD_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_plume_void, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->flags, breach, c->pvoid,
+    BD_CHECK(c->plume.nx <= 65535, "plume grid too large for the void-mask kernel");
+    const dim3 grid((unsigned)((c->plume.ny + PD_ROWS - 1) / PD_ROWS), (unsigned)c->plume.nx);
+    hipLaunchKernelGGL(k_plume_void, grid, dim3(64, PD_ROWS), 0, c->stream, c->flags, breach, c->pvoid,
                        c->g.nx, c->g.ny, c->g.nz, c->plume.nx, c->plume.ny, c->plume.nz,
                        gap ? ph.air_gap_k0 : 0, gap ? ph.air_gap_k1 : 0);
     BD_HIP(hipGetLastError());
     return BD_OK;
 }
 
+// plume.py:329 point injection of the cells just removed.  The removal marks are
+// scanned 16 bytes per load (most chunks hold none); a chunk that holds some adds
+// its cells in order.
+__device__ __forceinline__ void inject_cell(float *__restrict__ P, int64_t c, int Nx, int Ny, int Nz, int Px, int Py,
+                                            int Pz, float amount, int same)
+{
+    if (same) {
+        P[c] += amount;  // plume.py:329 raw_field[newly] += delta_mu (one thread per cell)
+    } else {
+        const int k = (int)(c % Nz);
+        const int64_t r = c / Nz;
+        const int j = (int)(r % Ny), i = (int)(r / Ny);
+        const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
+        unsafeAtomicAdd(&P[pc], amount);
+    }
+}
+
 __global__ void k_inject_point(const uint8_t *__restrict__ rem, float *__restrict__ P, int Nx, int Ny, int Nz,
                                int Px, int Py, int Pz, float amount, int same)
 {
     const int64_t N = (int64_t)Nx * Ny * Nz;
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
-        if (!rem[c]) continue;
-        if (same) {
-            P[c] += amount;  // plume.py:329 raw_field[newly] += delta_mu
-        } else {
-            const int k = (int)(c % Nz);
-            const int64_t r = c / Nz;
-            const int j = (int)(r % Ny), i = (int)(r / Ny);
-            const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
-            unsafeAtomicAdd(&P[pc], amount);
-        }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n16 = N / 16;  // rem is 16-byte aligned (the launcher checks)
+    const uint4 *r16 = (const uint4 *)rem;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n16; q += stride) {
+        const uint4 w = r16[q];
+        if (!(w.x | w.y | w.z | w.w)) continue;
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        for (int e = 0; e < 16; ++e)
+            if ((ws[e >> 2] >> (8 * (e & 3))) & 0xffu) inject_cell(P, q * 16 + e, Nx, Ny, Nz, Px, Py, Pz, amount, same);
     }
+    for (int64_t c = n16 * 16 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += stride)
+        if (rem[c]) inject_cell(P, c, Nx, Ny, Nz, Px, Py, Pz, amount, same);
 }
 
 __global__ void k_mask_void(float *__restrict__ P, const uint8_t *__restrict__ vm, int64_t n)
@@ -332,7 +405,8 @@ int launch_plume_inject(Ctx *c)
         BD_TRY(splat(c, c->kb, c->nkb, c->rb));
     } else {
         const double ratio = same ? 1.0 : (double)c->pvox / (double)c->nvox;
-        hipLaunchKernelGGL(k_inject_point, dim3(ew_blocks(c->nvox)), dim3(256), 0, c->stream, c->to_remove, c->P,
+        BD_CHECK((uintptr_t)c->to_remove % 16 == 0, "plume injection: removal marks are not 16-byte aligned");
+        hipLaunchKernelGGL(k_inject_point, dim3(ew_blocks((c->nvox + 15) / 16)), dim3(256), 0, c->stream, c->to_remove, c->P,
                            c->g.nx, c->g.ny, c->g.nz, p.nx, p.ny, p.nz, (float)(p.delta_mu * ratio), same ? 1 : 0);
         BD_HIP(hipGetLastError());
     }

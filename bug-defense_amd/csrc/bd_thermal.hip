@@ -591,6 +591,8 @@ static void fill_args(Ctx *c, SubArgs &a, int is_active, int flags)
 
 // Pick the tile shape once per context by timing the candidates on this grid
 // (a sub-step is idempotent only on scratch buffers, so tuning runs on T2 -> staging).
+constexpr int TUNE_REPS = 6;  // timed launches per shape (after one warm-up launch)
+
 static int autotune(Ctx *c)
 {
     if (c->tuned) return BD_OK;
@@ -611,21 +613,24 @@ static int autotune(Ctx *c)
         }
         return BD_OK;
     }
-    // Staging: T [0, 4n), H [4n, 8n), chi [8n, 12n), flags [12n, 13n), and the
-    // pair kernel's T output [13n, 17n) -- scratch copies, so the tuning launches
-    // run the epoch's per-cell work (plateau, char) and leave the state
-    // untouched.  Sized once, before any pointer into it is taken.
+    // Staging: T [0, 4n), H [4n, 8n), chi [8n, 12n), flags [12n, 13n), the
+    // pair kernel's T output [13n, 17n) and Omega [17n, 21n) -- scratch copies, so
+    // the tuning launches run the epoch's per-cell work (plateau, char, Arrhenius)
+    // and leave the state untouched.  Sized once, before any pointer into it is taken.
     const size_t n = (size_t)c->nvox;
-    BD_TRY(ensure_staging(c, n * 17));
+    BD_TRY(ensure_staging(c, n * 21));
     float *Tscr = (float *)c->staging;
     float *Hscr = (float *)((char *)c->staging + n * 4);
     float *Cscr = (float *)((char *)c->staging + n * 8);
     uint8_t *Fscr = (uint8_t *)((char *)c->staging + n * 12);
+    float *Oscr = (float *)((char *)c->staging + n * 17);
     const bool chr = c->chi && c->phys.has_char;
+    const bool arr = c->Om && c->phys.has_arrhenius;
     BD_HIP(hipMemcpyAsync(Tscr, c->T, n * 4, hipMemcpyDeviceToDevice, c->stream));
     BD_HIP(hipMemcpyAsync(Hscr, c->H, n * 4, hipMemcpyDeviceToDevice, c->stream));
     BD_HIP(hipMemcpyAsync(Fscr, c->flags, n, hipMemcpyDeviceToDevice, c->stream));
     if (chr) BD_HIP(hipMemcpyAsync(Cscr, c->chi, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    if (arr) BD_HIP(hipMemcpyAsync(Oscr, c->Om, n * 4, hipMemcpyDeviceToDevice, c->stream));
     const int bys[] = {2, 4, 8};
     const int xcs[] = {8, 16, 32, 64};
     float tms[3][4];
@@ -641,25 +646,26 @@ static int autotune(Ctx *c)
             a.flags = Fscr;
             a.chi = Cscr;
             a.chr = chr ? 1 : 0;
-            a.arr = 0;
+            a.Om = Oscr;
+            a.arr = arr ? 1 : 0;
             float ms = 0.f;
             BD_TRY(launch_one(c, a, bys[ib]));  // warm
             BD_HIP(hipEventRecord(c->ev0, c->stream));
-            for (int r = 0; r < 3; ++r) BD_TRY(launch_one(c, a, bys[ib]));
+            for (int r = 0; r < TUNE_REPS; ++r) BD_TRY(launch_one(c, a, bys[ib]));
             BD_HIP(hipEventRecord(c->ev1, c->stream));
             BD_HIP(hipEventSynchronize(c->ev1));
             BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
             tms[ib][ix] = ms;
             best = std::min(best, ms);
         }
-    // among shapes within 3 % of the fastest, the tallest tile (fewest y-halo
+    // among shapes within 1 % of the fastest, the tallest tile (fewest y-halo
     // re-reads of T and k), then the longest x march: timing noise does not
     // flip the choice between runs
     int bb = 4, bx = 16;
     for (int ib = 2; ib >= 0; --ib) {
         int got = -1;
         for (int ix = 3; ix >= 0; --ix)
-            if (tms[ib][ix] <= 1.03f * best) {
+            if (tms[ib][ix] <= 1.01f * best) {
                 got = ix;
                 break;
             }
@@ -690,15 +696,16 @@ static int autotune(Ctx *c)
             a.flags = Fscr;
             a.chi = Cscr;
             a.chr = chr ? 1 : 0;
-            a.arr = 0;
+            a.Om = Oscr;
+            a.arr = arr ? 1 : 0;
             float ms = 0.f;
             BD_TRY(launch_two(c, a, by, xc));
             BD_HIP(hipEventRecord(c->ev0, c->stream));
-            for (int r = 0; r < 3; ++r) BD_TRY(launch_two(c, a, by, xc));
+            for (int r = 0; r < TUNE_REPS; ++r) BD_TRY(launch_two(c, a, by, xc));
             BD_HIP(hipEventRecord(c->ev1, c->stream));
             BD_HIP(hipEventSynchronize(c->ev1));
             BD_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            if (c->tune_log) std::fprintf(stderr, "tune pair %dx%d: %.1f us\n", by, xc, ms * 1000.f / 3.f);
+            if (c->tune_log) std::fprintf(stderr, "tune pair %dx%d: %.1f us\n", by, xc, ms * 1000.f / TUNE_REPS);
             if (ms < best2) {
                 best2 = ms;
                 bb2 = by;
@@ -707,14 +714,14 @@ static int autotune(Ctx *c)
         }
     c->tile2_by = bb2;
     c->tile2_xc = bx2;
-    // three one-step vs three two-step launches
+    // TUNE_REPS one-step vs TUNE_REPS two-step launches
     c->use_two = c->two_step_allowed && (c->two_step_forced || best2 < 2.f * best);
     if (c->tune_log) {
         for (int ib = 0; ib < 3; ++ib)
             for (int ix = 0; ix < 4; ++ix)
-                std::fprintf(stderr, "tune %dx%d: %.1f us\n", bys[ib], xcs[ix], tms[ib][ix] * 1000.f / 3.f);
+                std::fprintf(stderr, "tune %dx%d: %.1f us\n", bys[ib], xcs[ix], tms[ib][ix] * 1000.f / TUNE_REPS);
         std::fprintf(stderr, "tune pair best %dx%d: %.1f us per launch (two sub-steps)%s\n", bb2, bx2,
-                     best2 * 1000.f / 3.f, c->use_two ? ", used" : "");
+                     best2 * 1000.f / TUNE_REPS, c->use_two ? ", used" : "");
     }
     if (c->use_two) {
         // the second H / flags buffers exist before any epoch is captured
