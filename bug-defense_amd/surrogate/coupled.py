@@ -141,7 +141,9 @@ class SurrogateCoupledEngine:
         t0 = self._mark("windows", t0)
 
         def low_photon_mc(wait):
-            (ctx.mc_run if wait else ctx.mc_launch)(noisy_beam(beam, lo))
+            # only a refining network reads the run's fluence; otherwise it scores absorbed
+            # weight alone (its Q gives the block totals), with no path-length records to add
+            (ctx.mc_run if wait else ctx.mc_launch)(noisy_beam(beam, lo, fluence=bool(mp)))
             ctx.sur_keep_stale()
             if self.mc_lo:
                 ctx.sur_keep_aux()
@@ -189,7 +191,7 @@ class SurrogateCoupledEngine:
                 self._predict(beam, into_q=False)
                 if self.noise_aware and not self.mc_lo:
                     # a second MC of the same state: the photon count of the epoch's own run, disjoint ids
-                    eng._ctx.mc_run(noisy_beam(beam, beam.n_photons, JUDGE_PHOTON_OFFSET))
+                    eng._ctx.mc_run(noisy_beam(beam, beam.n_photons, JUDGE_PHOTON_OFFSET, fluence=False))
                     eng._ctx.sur_keep_aux()
             d = eng.step_epoch()
             if compare:

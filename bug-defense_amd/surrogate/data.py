@@ -79,12 +79,13 @@ def sample_beam(s, n_photons, seed, power=1.0):
 NOISY_PHOTON_OFFSET = 1 << 40   # photon ids of the low-photon input run: disjoint from the target run's
 
 
-def noisy_beam(beam, mc_photons, photon_offset=NOISY_PHOTON_OFFSET):
-    """The beam of an extra MC run of the same state: same key, disjoint photon ids."""
+def noisy_beam(beam, mc_photons, photon_offset=NOISY_PHOTON_OFFSET, fluence=True):
+    """The beam of an extra MC run of the same state: same key, disjoint photon ids.
+    With fluence False the run scores absorbed weight only (no path-length tally)."""
     b = B.Beam.from_buffer_copy(beam)
     b.n_photons = int(mc_photons)
     b.photon_offset = int(photon_offset)
-    b.score_fluence = 1
+    b.score_fluence = 1 if fluence else 0
     return b
 
 
