@@ -62,10 +62,6 @@ struct McArgs {
     // where path lengths added inline go: Lt[voxel * Ls] (the L half of AL, or a separate grid)
     float *__restrict__ Lt;
     int Ls;
-    // the launch's lanes for this run: blocks [blk0, blk0 + nlanes / 256); lane l of them traces
-    // photon l first, and photons from nlanes on are handed out by the counter
-    int blk0;
-    long long nlanes;
 };
 
 constexpr unsigned TAG = 0x6d63u;
@@ -301,34 +297,28 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
         wcur = __builtin_amdgcn_readfirstlane(wcur + cnt);
     };
 
-    // First photon of each lane: its own index among the run's lanes, with no atomic.  The
-    // rest come from the photon counter; every wave's first refill would otherwise queue on
-    // that one address.  The counter only grows, so once a refill sees it past the last photon
-    // (or the lanes cover every photon) the wave stops asking: its lanes end as their photons do.
-    const unsigned long long gl = (unsigned long long)((long long)((int)blockIdx.x - a.blk0) * 256 + threadIdx.x);
-    bool first = true;
-    bool drained = a.nlanes >= a.n_photons;
+    // Photons are handed out by the counter, so the lanes of blocks that are resident early
+    // take the work of blocks not yet dispatched (a launch of several runs, or concurrent
+    // launches, holds more blocks than the device).  The counter only grows: once a refill
+    // sees it past the last photon the wave stops asking, and its lanes end as their photons do.
+    bool drained = false;
     for (;;) {
         // ---- refill idle lanes
         bool need = !alive && !exhausted;
-        if (need && !first && drained) {
+        if (need && drained) {
             exhausted = true;
             need = false;
         }
         const unsigned long long mask = __ballot(need);
         if (mask) {
-            unsigned long long my = gl;
-            if (!first) {
-                const int cnt = __popcll(mask);
-                const int leader = __ffsll((long long)mask) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)cnt);
-                base = __shfl(base, leader);
-                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-                my = (unsigned long long)a.nlanes + base + (unsigned long long)rank;
-                if ((unsigned long long)a.nlanes + base + (unsigned long long)cnt >= (unsigned long long)a.n_photons)
-                    drained = true;
-            }
+            const int cnt = __popcll(mask);
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)cnt);
+            base = __shfl(base, leader);
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            const unsigned long long my = base + (unsigned long long)rank;
+            if (base + (unsigned long long)cnt >= (unsigned long long)a.n_photons) drained = true;
             if (need) {
                 if (my >= (unsigned long long)a.n_photons) {
                     exhausted = true;
@@ -412,7 +402,6 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
                 }
             }
         }
-        first = false;
         if (!__any(alive)) {
             if (__all(exhausted)) break;
             continue;
@@ -1089,7 +1078,6 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
         c->join_pending = true;
     }
     const int nblk = (int)mc_blocks(c, b->n_photons);
-    a.nlanes = (long long)nblk * 256;
     if (timed == 1) BD_HIP(hipEventRecord(c->ev0, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[0], c->stream, hipEventRecordExternal));
     hipLaunchKernelGGL(k_mc, dim3(nblk), dim3(256), 0, c->stream, a);
@@ -1184,8 +1172,6 @@ int launch_mc_gang(Ctx *const *cs, const bd_beam *const *beams, int n)
         Ctx *c = cs[i];
         BD_TRY(mc_prepare(c, beams[i], c->ep, g->h_args[i]));
         const int64_t nb = mc_blocks(c, beams[i]->n_photons);
-        g->h_args[i].blk0 = (int)m;  // the run's blocks are [m, m + nb)
-        g->h_args[i].nlanes = nb * 256;
         for (int64_t b = 0; b < nb; ++b) g->h_map[m++] = i;
         BD_HIP(hipEventRecord(c->ev4, c->stream));
         BD_HIP(hipStreamWaitEvent(g->stream, c->ev4, 0));
@@ -1324,7 +1310,6 @@ extern "C" int bd_mc_run_noalloc(const bd_grid *grid, const bd_beam *beam, const
     const int64_t want = (beam->n_photons + 255) / 256;
     const int64_t cap = (int64_t)ncu * 8;
     const int nblk = (int)(want < 1 ? 1 : (want < cap ? want : cap));
-    a.nlanes = (long long)nblk * 256;
     hipLaunchKernelGGL(k_mc, dim3(nblk), dim3(256), 0, st, a);
     BD_HIP(hipGetLastError());
     double acc[7];
