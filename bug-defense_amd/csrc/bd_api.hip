@@ -151,7 +151,10 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     BD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     for (int i = 0; i < 7; ++i) BD_HIP(hipEventCreate(&c->gev[i]));
     if (const char *e = std::getenv("BUGDEF_MC_BLOCKS_PER_CU")) c->mc_blocks_per_cu = std::max(0, std::atoi(e));
-    if (const char *e = std::getenv("BUGDEF_MC_DEFER_FLUENCE")) c->defer_L = std::atoi(e) != 0 ? 1 : 0;
+    if (const char *e = std::getenv("BUGDEF_MC_DEFER_FLUENCE")) {
+        const int v = std::atoi(e);
+        c->defer_L = v == 0 ? 0 : (v == 2 ? 2 : 1);
+    }
     if (const char *e = std::getenv("BUGDEF_MC_RECORDS_PER_PHOTON")) c->rec_per_photon = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("BUGDEF_SUBSTEP_PAIRS")) {
         c->two_step_allowed = std::atoi(e) != 0;

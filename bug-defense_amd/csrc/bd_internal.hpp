@@ -111,9 +111,10 @@ struct Ctx {
     uint2 *rec_b = nullptr;
     float *Lb[2] = {nullptr, nullptr};
     int lag = -1;
-    // path lengths to records: 2 (default) only in eager epochs, where their passes run beside the
-    // next transport; 1 in every launch; 0 never (BUGDEF_MC_DEFER_FLUENCE=1 / 0)
-    int defer_L = 2;
+    // path lengths to records: 1 (default) in every launch (eager epochs run their passes beside
+    // the next transport, other launches right after the kernel); 2 only in eager epochs; 0 never
+    // (BUGDEF_MC_DEFER_FLUENCE=1 / 2 / 0)
+    int defer_L = 1;
     int rec_per_photon = 512;            // record buffer size per photon (BUGDEF_MC_RECORDS_PER_PHOTON)
     void *staging = nullptr;             // f64 staging for conversions
     size_t staging_bytes = 0;
