@@ -189,7 +189,8 @@ int bd_destroy(bd_ctx *h)
                     c->P, c->P2, c->Pext, c->pvoid, c->to_remove, c->red, c->lat, c->counter, c->staging,
                     c->splat_tmp1, c->splat_tmp2, c->kb[0], c->kb[1], c->kb[2], c->kl[0], c->kl[1], c->kl[2],
                     c->Qs, c->Qsur, c->Qaux, c->Hb, c->flagsb, c->rec, c->rec_sorted, c->rec_tcnt,
-                    c->rec_tstart, c->rec_off, c->rec_cstart, c->rec_cmap, c->rec_b, c->Lb[0], c->Lb[1]};
+                    c->rec_tstart, c->rec_off, c->rec_cstart, c->rec_cmap, c->rec_b, c->Lb[0], c->Lb[1],
+                    c->led_part};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto &f : c->ds)

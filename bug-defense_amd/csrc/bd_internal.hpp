@@ -116,6 +116,7 @@ struct Ctx {
     // (BUGDEF_MC_DEFER_FLUENCE=1 / 2 / 0)
     int defer_L = 1;
     int rec_per_photon = 512;            // record buffer size per photon (BUGDEF_MC_RECORDS_PER_PHOTON)
+    double *led_part = nullptr;          // sub-step ledger slots, zero between calls
     void *staging = nullptr;             // f64 staging for conversions
     size_t staging_bytes = 0;
     void *host_pinned = nullptr;         // small pinned host block (64 doubles)

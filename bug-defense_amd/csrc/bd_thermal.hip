@@ -16,6 +16,7 @@
 // the y-halo re-reads inside one L2.
 #include "bd_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <utility>
@@ -41,7 +42,7 @@ struct SubArgs {
     float *__restrict__ chi;
     float *__restrict__ Om;
     float *__restrict__ Tpeak;
-    double *__restrict__ ledger;  // conducted energy accumulator, or null
+    double *__restrict__ ledger;  // conducted energy: LEDGER_SLOTS FP64 slots LEDGER_STRIDE apart, or null
     int Nx, Ny, Nz;
     int ntz, nty, ntx, xc;
     float idx2, idy2, idz2, dz, inv_h, h_eff;
@@ -198,14 +199,23 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Adds a block's conducted boundary energy to the ledger.  Only the cells at
-// z == 0 and z == Nz-1 carry a boundary flux, so only the lanes holding those
+// Conducted-energy ledger slots of the sub-step launches: a z-face block adds
+// into slot t mod LEDGER_SLOTS, the slots a few cache lines apart, and the call
+// sums them once after its launches.  A single accumulator took one atomic from
+// every z-face block of a launch (two thirds of them at 192 z cells), all queued
+// on one L2 line.
+constexpr int LEDGER_SLOTS = 64;
+constexpr int LEDGER_STRIDE = 64;  // doubles: 512 bytes between slots
+
+// Adds a block's conducted boundary energy to its ledger slot.  Only the cells
+// at z == 0 and z == Nz-1 carry a boundary flux, so only the lanes holding those
 // two planes can be non-zero: each wave reads them out of its lanes (no LDS
 // tree, no barrier in blocks away from the z faces), the block sums its waves
 // in wave order and issues at most one atomic.  zlane0 is the z of lane 0;
 // lanes below `lmin` are halo lanes whose econd is zero anyway.
 template <int BY>
-__device__ __forceinline__ void ledger_flush(const SubArgs &a, double econd, int zlane0, int lmin, int wy, int lane)
+__device__ __forceinline__ void ledger_flush(const SubArgs &a, double econd, int zlane0, int lmin, int wy, int lane,
+                                             int t)
 {
     const int l_top = -zlane0, l_bot = a.Nz - 1 - zlane0;
     const bool has_top = l_top >= lmin && l_top < 64, has_bot = l_bot >= lmin && l_bot < 64 && l_bot != l_top;
@@ -219,7 +229,7 @@ __device__ __forceinline__ void ledger_flush(const SubArgs &a, double econd, int
     if (wy == 0 && lane == 0) {
         double s = 0.0;
         for (int i = 0; i < BY; ++i) s += red[i];
-        if (s != 0.0) unsafeAtomicAdd(a.ledger, s * a.vol_dt);
+        if (s != 0.0) unsafeAtomicAdd(&a.ledger[(t % LEDGER_SLOTS) * LEDGER_STRIDE], s * a.vol_dt);
     }
 }
 
@@ -364,7 +374,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep(SubArgs a)
         Tp = Tn2; kp = kn2;
         buf ^= 1;
     }
-    if (a.ledger) ledger_flush<BY>(a, econd, z - lane, 0, wy, lane);
+    if (a.ledger) ledger_flush<BY>(a, econd, z - lane, 0, wy, lane, t);
 }
 
 // Two sub-steps per launch (temporal blocking).  A block owns a 60(z) x
@@ -507,7 +517,7 @@ __global__ __launch_bounds__(64 * BY) void k_substep2(SubArgs a)
         kbp = kb;
         kb = kb == 2 ? 0 : kb + 1;
     }
-    if (a.ledger) ledger_flush<BY>(a, econd, z - lane, 2, wy, lane);
+    if (a.ledger) ledger_flush<BY>(a, econd, z - lane, 2, wy, lane, t);
 }
 
 static int launch_one(Ctx *c, SubArgs &a, int by)
@@ -591,6 +601,15 @@ static void fill_args(Ctx *c, SubArgs &a, int is_active, int flags)
 
 // Pick the tile shape once per context by timing the candidates on this grid
 // (a sub-step is idempotent only on scratch buffers, so tuning runs on T2 -> staging).
+// *acc += the ledger slots (one wave, a fixed order), clearing them.
+__global__ __launch_bounds__(64) void k_ledger_sum(double *__restrict__ part, double *__restrict__ acc)
+{
+    double v = part[threadIdx.x * LEDGER_STRIDE];
+    part[threadIdx.x * LEDGER_STRIDE] = 0.0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (threadIdx.x == 0) *acc += v;
+}
+
 constexpr int TUNE_REPS = 6;  // timed launches per shape (after one warm-up launch)
 
 static int autotune(Ctx *c)
@@ -753,7 +772,11 @@ int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger)
     fill_args(c, a, is_active, flags);
     a.Om = c->Om;
     a.Tpeak = c->Tpeak;
-    a.ledger = ledger ? c->red + 8 : nullptr;  // red[8]: conducted energy of this call
+    if (ledger) {
+        // the ledger slots (zero between calls), summed into red[8] after the launches
+        BD_TRY(ensure(c, (void **)&c->led_part, sizeof(double) * LEDGER_SLOTS * LEDGER_STRIDE));
+        a.ledger = c->led_part;
+    }
     // pairs of sub-steps in the two-step kernel, an even number of them so the
     // H / flags buffers end where they started; the rest one at a time
     const int n_two = c->use_two ? 2 * (n / 4) : 0;
@@ -779,6 +802,11 @@ int launch_substeps(Ctx *c, int n, int is_active, int flags, bool ledger)
         a.peak = ((flags & BD_SUB_PEAK) && done == n - 1) ? 1 : 0;
         BD_TRY(launch_one(c, a, c->tile_by));
         std::swap(c->T, c->T2);
+    }
+    if (ledger && n > 0) {
+        // red[8] += the slots, which are left zero
+        hipLaunchKernelGGL(k_ledger_sum, dim3(1), dim3(64), 0, c->stream, c->led_part, c->red + 8);
+        BD_HIP(hipGetLastError());
     }
     return BD_OK;
 }
