@@ -161,6 +161,7 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
         c->two_step_forced = std::atoi(e) == 2;
     }
     if (const char *e = std::getenv("BUGDEF_TUNE_LOG")) c->tune_log = std::atoi(e) != 0;
+    if (const char *e = std::getenv("BUGDEF_PLUME_PAIRS")) c->plume_pairs = std::atoi(e) != 0;
     // a fixed one-step tile "BYxXC" (BY 2, 4 or 8; XC >= 1) in place of the timed choice, e.g. for
     // counter runs, where the profiler's per-dispatch overhead distorts the tuning launches
     if (const char *e = std::getenv("BUGDEF_SUBSTEP_TILE")) {

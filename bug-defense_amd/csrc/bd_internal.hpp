@@ -117,6 +117,7 @@ struct Ctx {
     int defer_L = 1;
     int rec_per_photon = 512;            // record buffer size per photon (BUGDEF_MC_RECORDS_PER_PHOTON)
     double *led_part = nullptr;          // sub-step ledger slots, zero between calls
+    bool plume_pairs = true;             // paired plume diffusion rows where they fit (BUGDEF_PLUME_PAIRS)
     void *staging = nullptr;             // f64 staging for conversions
     size_t staging_bytes = 0;
     void *host_pinned = nullptr;         // small pinned host block (64 doubles)

@@ -33,6 +33,20 @@ struct PlumeStep {
     float idx2, idy2, idz2, inv_dz, dt, Dxy, Dz, vz, decay;
 };
 
+// The diffusion update of one void interior cell from its neighbourhood
+// (plume.py:18-80, FP32): a neighbour that is not void reflects the cell's own value.
+// Contraction is off, so both row kernels below produce the same bits.
+__device__ __forceinline__ float plume_cell(const PlumeStep &p, float Cc, float Cw, float Ce, float Cs, float Cq,
+                                            float Cb, float Ct)
+{
+#pragma clang fp contract(off)
+    const float diff = p.Dxy * ((Cw - 2.f * Cc + Ce) * p.idx2 + (Cs - 2.f * Cc + Cq) * p.idy2) +
+                       p.Dz * ((Cb - 2.f * Cc + Ct) * p.idz2);
+    const float adv = p.vz > 0.f ? -p.vz * (Cc - Ct) * p.inv_dz : 0.f;
+    const float v = (Cc + p.dt * (diff + adv)) * p.decay;
+    return fmaxf(v, 0.f);
+}
+
 // The step on the z row (i, j), its cells taken 64 at a time by the lanes of one
 // wave, so every neighbour load is one coalesced row access.  Indices are 32-bit
 // (plume grids have fewer than 2^31 cells; the launcher checks).
@@ -49,17 +63,58 @@ __device__ __forceinline__ void plume_row(const PlumeStep &p, int i, int j, int 
             continue;
         }
         const float Cc = p.C[c];
-        const float Cw = p.vm[c - pl] ? p.C[c - pl] : Cc;
-        const float Ce = p.vm[c + pl] ? p.C[c + pl] : Cc;
-        const float Cs = p.vm[c - Nz] ? p.C[c - Nz] : Cc;
-        const float Cq = p.vm[c + Nz] ? p.C[c + Nz] : Cc;
-        const float Cb = p.vm[c - 1] ? p.C[c - 1] : Cc;
-        const float Ct = p.vm[c + 1] ? p.C[c + 1] : Cc;
-        const float diff = p.Dxy * ((Cw - 2.f * Cc + Ce) * p.idx2 + (Cs - 2.f * Cc + Cq) * p.idy2) +
-                           p.Dz * ((Cb - 2.f * Cc + Ct) * p.idz2);
-        const float adv = p.vz > 0.f ? -p.vz * (Cc - Ct) * p.inv_dz : 0.f;
-        const float v = (Cc + p.dt * (diff + adv)) * p.decay;
-        p.Cn[c] = fmaxf(v, 0.f);
+        p.Cn[c] = plume_cell(p, Cc, p.vm[c - pl] ? p.C[c - pl] : Cc, p.vm[c + pl] ? p.C[c + pl] : Cc,
+                             p.vm[c - Nz] ? p.C[c - Nz] : Cc, p.vm[c + Nz] ? p.C[c + Nz] : Cc,
+                             p.vm[c - 1] ? p.C[c - 1] : Cc, p.vm[c + 1] ? p.C[c + 1] : Cc);
+    }
+}
+
+// The same step for rows whose length is a multiple of 128: a lane takes the
+// cells z = k0 and k0 + 64 together, first their void flags, then both
+// neighbourhoods, each set of loads issued at once, so the pair waits on memory
+// twice instead of twice per cell.  A cell that is not void or not interior
+// gathers its own cell again (a cache hit), and a select, not a branch, writes 0
+// for it, so no load is left behind a branch.
+__device__ __forceinline__ void plume_row2(const PlumeStep &p, int i, int j, int lane)
+{
+    const int Ny = p.Ny, Nz = p.Nz;
+    const int pl = Ny * Nz;
+    const int row = (i * Ny + j) * Nz;
+    const bool edge_row = i == 0 || i == p.Nx - 1 || j == 0 || j == Ny - 1;
+    for (int k0 = lane; k0 < Nz; k0 += 128) {
+        int c[2];
+        uint8_t vc[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            c[e] = row + k0 + 64 * e;
+            vc[e] = p.vm[c[e]];
+        }
+        bool act[2];
+        uint8_t vn[2][6];
+        float cc[2], cn[2][6];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int k = k0 + 64 * e;
+            act[e] = vc[e] && !edge_row && k > 0 && k < Nz - 1;
+            const int x = c[e];
+            const int id[6] = {act[e] ? x - pl : x, act[e] ? x + pl : x, act[e] ? x - Nz : x,
+                               act[e] ? x + Nz : x, act[e] ? x - 1 : x,  act[e] ? x + 1 : x};
+            cc[e] = p.C[x];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                vn[e][q] = p.vm[id[q]];
+                cn[e][q] = p.C[id[q]];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float Cc = cc[e];
+            float nb[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) nb[q] = vn[e][q] ? cn[e][q] : Cc;
+            const float v = plume_cell(p, Cc, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5]);
+            p.Cn[c[e]] = act[e] ? v : 0.f;
+        }
     }
 }
 
@@ -68,10 +123,13 @@ __device__ __forceinline__ void plume_row(const PlumeStep &p, int i, int j, int 
 // x, blockIdx.x = group of y rows), so no lane divides a linear index.
 constexpr int PD_ROWS = 4;
 
+template <bool PAIRS>
 __global__ __launch_bounds__(64 * PD_ROWS) void k_plume_diffuse(PlumeStep p)
 {
     const int j = blockIdx.x * PD_ROWS + threadIdx.y;
-    if (j < p.Ny) plume_row(p, blockIdx.y, j, threadIdx.x);
+    if (j >= p.Ny) return;
+    if (PAIRS) plume_row2(p, blockIdx.y, j, threadIdx.x);
+    else plume_row(p, blockIdx.y, j, threadIdx.x);
 }
 
 static int ew_blocks(int64_t n) { return blocks_for(n, 256) > 8192 ? 8192 : blocks_for(n, 256); }
@@ -115,10 +173,13 @@ static int plume_step_args(Ctx *c, double dt, double decay, PlumeStep &ps)
 static int launch_plume_steps(Ctx *c, int n, PlumeStep ps)
 {
     const dim3 grid((unsigned)((ps.Ny + PD_ROWS - 1) / PD_ROWS), (unsigned)ps.Nx);
+    // rows of a multiple of 128 cells take the paired kernel (BUGDEF_PLUME_PAIRS=0: never)
+    const bool pairs = ps.Nz % 128 == 0 && c->plume_pairs;
     for (int s = 0; s < n; ++s) {
         ps.C = c->P;
         ps.Cn = c->P2;
-        hipLaunchKernelGGL(k_plume_diffuse, grid, dim3(64, PD_ROWS), 0, c->stream, ps);
+        if (pairs) hipLaunchKernelGGL(k_plume_diffuse<true>, grid, dim3(64, PD_ROWS), 0, c->stream, ps);
+        else hipLaunchKernelGGL(k_plume_diffuse<false>, grid, dim3(64, PD_ROWS), 0, c->stream, ps);
         BD_HIP(hipGetLastError());
         std::swap(c->P, c->P2);
     }

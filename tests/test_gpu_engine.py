@@ -96,6 +96,50 @@ def test_plume_kernels_parity(oracle):
     np.testing.assert_array_equal(raw_g, raw_o)  # same FP32 accumulation order: bit-exact
 
 
+def test_paired_plume_rows_match_single_rows(oracle):
+    """Rows of a multiple of 128 cells take the paired diffusion kernel (two cells per lane,
+    loads issued together, a select for cells that are not void interior ones).  Against the
+    one-cell-at-a-time kernel (BUGDEF_PLUME_PAIRS=0, read when the context is made): the same
+    bits after 12 steps on a random field and void mask; and one paired step against the
+    oracle within FP32 round-off (the bar of test_plume_kernels_parity)."""
+    import os
+    from simulation.plume import MultispectralPlumeField
+    rng = np.random.default_rng(11)
+    shape = (13, 11, 256)
+    C = rng.uniform(0, 2.0, shape).astype(np.float32)
+    void = rng.random(shape) < 0.7
+    dt = 2e-5
+
+    def field(pairs):
+        old = os.environ.get("BUGDEF_PLUME_PAIRS")
+        os.environ["BUGDEF_PLUME_PAIRS"] = pairs
+        try:
+            p = MultispectralPlumeField(shape=shape, model='diffusion', D_plume=0.5, v_drift=10.0,
+                                        blast_expansion=False)
+        finally:
+            if old is None:
+                del os.environ["BUGDEF_PLUME_PAIRS"]
+            else:
+                os.environ["BUGDEF_PLUME_PAIRS"] = old
+        p.raw_field = C
+        return p
+
+    paired, single = field("1"), field("0")
+    paired.diffuse_substep(void, dt_sub=dt)
+    Co = np.empty_like(C)
+    decay = float(np.exp(-dt / paired.tau_linger))
+    oracle.plume_diffuse(C, Co, void.astype(np.uint8), *shape, paired.dx, paired.dy, paired.dz, dt, paired.D_xy,
+                         paired.D_z, paired.v_drift, decay)
+    np.testing.assert_allclose(paired.raw_field, Co, rtol=2e-6, atol=1e-7)
+    single.diffuse_substep(void, dt_sub=dt)
+    for _ in range(11):
+        paired.diffuse_substep(void, dt_sub=dt)
+        single.diffuse_substep(void, dt_sub=dt)
+    a, b = paired.raw_field, single.raw_field
+    assert np.any(a > 0.0)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_plume_reference_fixtures_on_gpu():
     """tests/test_plume.py:110-179,230-266 of the reference on the device plume field."""
     from simulation.plume import MultispectralPlumeField
