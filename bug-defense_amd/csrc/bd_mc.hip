@@ -66,6 +66,7 @@ struct McArgs {
 
 constexpr unsigned TAG = 0x6d63u;
 constexpr unsigned REC_NONE = 0xffffffffu;
+constexpr int INTER_EVERY = 4;   // interaction iterations of a burst: every 4th (see mc_trace)
 constexpr int REC_CHUNK = 2048;  // record slots a wave takes at a time (one counter atomic)
 
 // A photon's stream is the sequence of Philox blocks (blk = 0, 1, 2, ...), each giving four
@@ -431,9 +432,19 @@ __device__ __forceinline__ void mc_trace(const McArgs &a)
             else if (sy <= sz) { axis = 1; s_exit = sy; }
             else { axis = 2; s_exit = sz; }
             s_exit = fmaxf(s_exit, 0.f);
+            // Interactions in batches.  Most events are chords, yet nearly every iteration of a
+            // 64-lane wave has some lane at an interaction, so the wave used to run both branch
+            // bodies every iteration.  A lane whose event is an interaction now waits (masked,
+            // its photon unchanged) until every INTER_EVERY-th iteration of the burst, or until
+            // half of the wave's live lanes wait, so most iterations run the chord body alone.
+            // Each photon's sequence of events is the same; only the interleaving changes.
+            const bool inter = mut > 1e-12f && mut * s_exit >= p.tau;
+            const int n_wait = __popcll(__ballot(inter));
+            const int n_live = __popcll(__ballot(true));  // every live lane, before any branch
+            if (inter && it % INTER_EVERY != INTER_EVERY - 1 && 2 * n_wait < n_live) continue;
             t_ev += 1;
             p.ev++;
-            if (mut > 1e-12f && mut * s_exit >= p.tau) {
+            if (inter) {
                 // ---- interaction inside the voxel (voxel_mc_3d.py:201-265)
                 const float s_col = p.tau / mut;
                 tally(v, 0.f, p.w * s_col);
