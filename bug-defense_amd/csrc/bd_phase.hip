@@ -127,10 +127,11 @@ __device__ __forceinline__ uint8_t phase_cell(const PhaseArgs &a, int64_t i, uin
 
 // Four consecutive cells per thread and iteration: their flags, T and latent heat
 // come in one vector load each, issued together before any branch (the banked
-// enthalpy only where a cell has some), so a quad waits on memory once; the cells are then updated in
-// order by phase_cell, and the new flags and removal marks go out as one 4-byte
-// store each.  Ablated cells keep their flags and get a zero mark.  The last
-// nvox % 4 cells are taken one at a time by the first threads of block 0.
+// enthalpy only where a cell has some), so a quad waits on memory once.  The
+// cells are then updated in order by phase_cell, and the new flags and removal
+// marks go out as one 4-byte store each.  Ablated cells keep their flags and get
+// a zero mark.  The last nvox % 4 cells are taken one at a time by the first
+// threads of block 0.
 __global__ __launch_bounds__(256) void k_phase(PhaseArgs a)
 {
     unsigned cnt = 0;
