@@ -174,7 +174,7 @@ int bd_create(const bd_grid *grid, bd_ctx **out)
     }
     BD_HIP(hipHostMalloc(&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault));
     BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + 8192)));
-    BD_TRY(ensure(c, (void **)&c->counter, sizeof(unsigned long long) * 16));
+    BD_TRY(ensure(c, (void **)&c->counter, sizeof(unsigned long long) * COUNTER_SLOTS));
     *out = h;
     return BD_OK;
 }

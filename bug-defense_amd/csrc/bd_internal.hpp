@@ -97,7 +97,10 @@ struct Ctx {
     double *red = nullptr;               // reduction accumulators (64 doubles) + 8192 doubles of partials
     double *lat = nullptr;               // lateral escape per depth
     int64_t lat_n = 0;
-    unsigned long long *counter = nullptr;  // [0] photon work counter, [1] n_new, [2] breach, [3] record cursor
+    // COUNTER_SLOTS 64-bit slots: [0] photon work counter, [1] n_new, [2] breach, [3] and [4] the
+    // record cursors of the two record buffers, [8] surrogate window bounds (two ints), [10] and
+    // [11] the legacy plume update's flags, [16, 22) the plume splat's bounding box
+    unsigned long long *counter = nullptr;
     uint2 *rec = nullptr;                // deferred path-length records of the transport
     uint2 *rec_sorted = nullptr;         // the records in tile order (record passes)
     int64_t rec_cap = 0;                 // records each of the two buffers holds
@@ -197,6 +200,8 @@ int launch_char(Ctx *c, double A, double EaR, double dt);
 // where the record passes of a transport launch run: after it on the context stream, or at the
 // next eager epoch, on the side stream beside that epoch's transport
 enum RecMode { REC_MAIN = 0, REC_LAG = 2 };
+constexpr int COUNTER_SLOTS = 32;
+constexpr int COUNTER_BBOX = 16;  // six slots
 int launch_mc(Ctx *c, const bd_beam *b, bd_mc_result *out);  // bd_mc_run: passes after the kernel
 int join_records(Ctx *c);   // the context stream waits for record passes left on the side stream
 int lag_finalize(Ctx *c);   // the lagged record passes (if any) run and their L lands in AL

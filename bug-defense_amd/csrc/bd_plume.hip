@@ -423,7 +423,8 @@ static int splat(Ctx *c, float *const *k3, const int *nk, const int *rr)
         set_error("plume splat kernels not uploaded (bd_set_plume_kernels)");
         return BD_ESTATE;
     }
-    unsigned long long *bb = c->counter + 4;
+    // the box has slots of its own: a lagged epoch's record cursor lives past this call
+    unsigned long long *bb = c->counter + COUNTER_BBOX;
     unsigned long long init[6] = {~0ull, 0ull, ~0ull, 0ull, ~0ull, 0ull};
     BD_HIP(hipMemcpyAsync(bb, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_bbox, dim3(ew_blocks(c->nvox)), dim3(256), 0, c->stream, c->to_remove, c->g.nx, c->g.ny,

@@ -308,6 +308,28 @@ def test_epoch_path_length_tally_lags_one_epoch_and_is_complete_on_read(schedule
         ei.close()
 
 
+def test_path_length_records_survive_the_blast_plume_splat():
+    """Regression: with a blast-expansion plume an epoch that removes cells splats the plume
+    through a bounding box kept in the context's counter slots, and the record cursor of the
+    epoch's second record buffer (the one every second eager epoch writes, whose passes run on
+    the next read) must not share those slots.  Four epochs that remove cells, then a read of the
+    tallies, against an engine adding path lengths inline (same photons)."""
+    plume = {'model': 'diffusion', 'blast_expansion': True, 'D_plume': 0.5, 'v_drift': 10.0}
+    ed, ei = _engine_pair(power=40.0, plume_closure=plume)
+    try:
+        removed = []
+        for _ in range(4):
+            ed.step_epoch()
+            ei.step_epoch()
+            removed.append(int(ed.last_diag.n_new))
+        assert removed[1] > 0 and removed[3] > 0, removed  # the splat ran in both even epochs
+        assert not np.any(np.asarray(ed.ablated) != np.asarray(ei.ablated))
+        _tallies_close(ed, ei)
+    finally:
+        ed.close()
+        ei.close()
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 1), (1, 1, 6), (3, 2, 1), (2, 5, 3)])
 def test_degenerate_grids_gpu_vs_oracle(oracle, shape):
     """One voxel, one column, one plane: the entrance split, exits through every
