@@ -94,7 +94,10 @@ struct Ctx {
     size_t splat_tmp_bytes = 0;
 
     // scratch
-    double *red = nullptr;               // reduction accumulators (64 doubles) + 8192 doubles of partials
+    // reduction accumulators (64 doubles) + 8192 doubles of partials from red + 64: [8] conducted
+    // energy (sub-step ledger sum), [9] ablated energy (phase), [16, 24) the transport's scalar
+    // tallies, [24, 28) plume diagnostics, [40] masked reductions, [48, 54) surrogate error terms
+    double *red = nullptr;
     double *lat = nullptr;               // lateral escape per depth
     int64_t lat_n = 0;
     // COUNTER_SLOTS 64-bit slots: [0] photon work counter, [1] n_new, [2] breach, [3] and [4] the
