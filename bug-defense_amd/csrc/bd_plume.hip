@@ -23,6 +23,49 @@ __device__ __forceinline__ int cmap(int i, int n_from, int n_to)
     return (int)(((int64_t)(2 * i + 1) * n_to) / (2 * (int64_t)n_from));
 }
 
+// The same map in 32-bit unsigned arithmetic (W32), exact where every
+// (2 i + 1) n_to and every voxel index stays below 2^32 (map32_ok).  A 64-bit
+// division by a runtime value is a software routine of some 150 instructions; a
+// gather of 16 cells mapped that way kept a wave busy for microseconds.
+template <bool W32>
+__device__ __forceinline__ int cmapw(int i, int n_from, int n_to)
+{
+    if (W32) return (int)(((unsigned)(2 * i + 1) * (unsigned)n_to) / (2u * (unsigned)n_from));
+    return cmap(i, n_from, n_to);
+}
+
+// (i, j, k) of voxel c of an (Nx, Ny, Nz) grid, z fastest
+template <bool W32>
+__device__ __forceinline__ void split_ijk(int64_t c, int Ny, int Nz, int &i, int &j, int &k)
+{
+    if (W32) {
+        const unsigned u = (unsigned)c, r = u / (unsigned)Nz, ii = r / (unsigned)Ny;
+        k = (int)(u - r * (unsigned)Nz);
+        j = (int)(r - ii * (unsigned)Ny);
+        i = (int)ii;
+    } else {
+        k = (int)(c % Nz);
+        const int64_t r = c / Nz;
+        j = (int)(r % Ny);
+        i = (int)(r / Ny);
+    }
+}
+
+// the plume cell under thermal voxel (i, j, k)
+template <bool W32>
+__device__ __forceinline__ int64_t plume_of(int i, int j, int k, int Nx, int Ny, int Nz, int Px, int Py, int Pz)
+{
+    return ((int64_t)cmapw<W32>(i, Nx, Px) * Py + cmapw<W32>(j, Ny, Py)) * Pz + cmapw<W32>(k, Nz, Pz);
+}
+
+// both directions of the map (thermal -> plume and plume -> thermal) and every voxel index
+static bool map32_ok(const Ctx *c)
+{
+    const int64_t m = std::max({(int64_t)c->g.nx, (int64_t)c->g.ny, (int64_t)c->g.nz, (int64_t)c->plume.nx,
+                                (int64_t)c->plume.ny, (int64_t)c->plume.nz});
+    return c->nvox < (int64_t(1) << 32) && (2 * m + 1) * m < (int64_t(1) << 32);
+}
+
 // One explicit step of the plume diffusion (plume.py:18-80, FP32 arithmetic)
 // on the plume grid (Nx, Ny, Nz), z fastest: C in, Cn out, vm the void mask.
 struct PlumeStep {
@@ -213,6 +256,7 @@ __global__ void k_gap_any(const uint8_t *__restrict__ flags, int Nx, int Ny, int
 // The plume void mask, one z row of plume cells per wave (blockIdx.y = I, blockIdx.x =
 // group of PD_ROWS J rows): the thermal cell under each plume cell's centre is void,
 // or the cell is in the air gap once the gap is breached (coupled_ablation.py:588,624).
+template <bool W32>
 __global__ __launch_bounds__(64 * PD_ROWS) void k_plume_void(const uint8_t *__restrict__ flags,
                                                              const unsigned long long *__restrict__ breach,
                                                              uint8_t *__restrict__ vm, int Nx, int Ny, int Nz, int Px,
@@ -221,10 +265,10 @@ __global__ __launch_bounds__(64 * PD_ROWS) void k_plume_void(const uint8_t *__re
     const int I = blockIdx.y, J = blockIdx.x * PD_ROWS + threadIdx.y;
     if (J >= Py) return;
     const bool br = breach && *breach;
-    const int64_t col = ((int64_t)cmap(I, Px, Nx) * Ny + cmap(J, Py, Ny)) * Nz;
+    const int64_t col = ((int64_t)cmapw<W32>(I, Px, Nx) * Ny + cmapw<W32>(J, Py, Ny)) * Nz;
     uint8_t *out = vm + ((int64_t)I * Py + J) * Pz;
     for (int K = threadIdx.x; K < Pz; K += 64) {
-        const int k = cmap(K, Pz, Nz);
+        const int k = cmapw<W32>(K, Pz, Nz);
         const bool v = (flags[col + k] & F_ABL) || (br && k >= k0 && k < k1);
         out[K] = v ? 1 : 0;
     }
@@ -245,7 +289,8 @@ int launch_plume_void(Ctx *c)
     }
     BD_CHECK(c->plume.nx <= 65535, "plume grid too large for the void-mask kernel");
     const dim3 grid((unsigned)((c->plume.ny + PD_ROWS - 1) / PD_ROWS), (unsigned)c->plume.nx);
-    hipLaunchKernelGGL(k_plume_void, grid, dim3(64, PD_ROWS), 0, c->stream, c->flags, breach, c->pvoid,
+    hipLaunchKernelGGL(map32_ok(c) ? k_plume_void<true> : k_plume_void<false>, grid, dim3(64, PD_ROWS), 0, c->stream,
+                       c->flags, breach, c->pvoid,
                        c->g.nx, c->g.ny, c->g.nz, c->plume.nx, c->plume.ny, c->plume.nz,
                        gap ? ph.air_gap_k0 : 0, gap ? ph.air_gap_k1 : 0);
     BD_HIP(hipGetLastError());
@@ -255,20 +300,20 @@ int launch_plume_void(Ctx *c)
 // plume.py:329 point injection of the cells just removed.  The removal marks are
 // scanned 16 bytes per load (most chunks hold none); a chunk that holds some adds
 // its cells in order.
+template <bool W32>
 __device__ __forceinline__ void inject_cell(float *__restrict__ P, int64_t c, int Nx, int Ny, int Nz, int Px, int Py,
                                             int Pz, float amount, int same)
 {
     if (same) {
         P[c] += amount;  // plume.py:329 raw_field[newly] += delta_mu (one thread per cell)
     } else {
-        const int k = (int)(c % Nz);
-        const int64_t r = c / Nz;
-        const int j = (int)(r % Ny), i = (int)(r / Ny);
-        const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
-        unsafeAtomicAdd(&P[pc], amount);
+        int i, j, k;
+        split_ijk<W32>(c, Ny, Nz, i, j, k);
+        unsafeAtomicAdd(&P[plume_of<W32>(i, j, k, Nx, Ny, Nz, Px, Py, Pz)], amount);
     }
 }
 
+template <bool W32>
 __global__ void k_inject_point(const uint8_t *__restrict__ rem, float *__restrict__ P, int Nx, int Ny, int Nz,
                                int Px, int Py, int Pz, float amount, int same)
 {
@@ -281,10 +326,11 @@ __global__ void k_inject_point(const uint8_t *__restrict__ rem, float *__restric
         if (!(w.x | w.y | w.z | w.w)) continue;
         const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
         for (int e = 0; e < 16; ++e)
-            if ((ws[e >> 2] >> (8 * (e & 3))) & 0xffu) inject_cell(P, q * 16 + e, Nx, Ny, Nz, Px, Py, Pz, amount, same);
+            if ((ws[e >> 2] >> (8 * (e & 3))) & 0xffu)
+                inject_cell<W32>(P, q * 16 + e, Nx, Ny, Nz, Px, Py, Pz, amount, same);
     }
     for (int64_t c = n16 * 16 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += stride)
-        if (rem[c]) inject_cell(P, c, Nx, Ny, Nz, Px, Py, Pz, amount, same);
+        if (rem[c]) inject_cell<W32>(P, c, Nx, Ny, Nz, Px, Py, Pz, amount, same);
 }
 
 __global__ void k_mask_void(float *__restrict__ P, const uint8_t *__restrict__ vm, int64_t n)
@@ -468,8 +514,9 @@ int launch_plume_inject(Ctx *c)
     } else {
         const double ratio = same ? 1.0 : (double)c->pvox / (double)c->nvox;
         BD_CHECK((uintptr_t)c->to_remove % 16 == 0, "plume injection: removal marks are not 16-byte aligned");
-        hipLaunchKernelGGL(k_inject_point, dim3(ew_blocks((c->nvox + 15) / 16)), dim3(256), 0, c->stream, c->to_remove, c->P,
-                           c->g.nx, c->g.ny, c->g.nz, p.nx, p.ny, p.nz, (float)(p.delta_mu * ratio), same ? 1 : 0);
+        hipLaunchKernelGGL(map32_ok(c) ? k_inject_point<true> : k_inject_point<false>,
+                           dim3(ew_blocks((c->nvox + 15) / 16)), dim3(256), 0, c->stream, c->to_remove, c->P, c->g.nx,
+                           c->g.ny, c->g.nz, p.nx, p.ny, p.nz, (float)(p.delta_mu * ratio), same ? 1 : 0);
         BD_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_mask_void, dim3(ew_blocks(c->pvox)), dim3(256), 0, c->stream, c->P, c->pvoid, c->pvox);
@@ -555,6 +602,7 @@ int launch_plume_update_sep(Ctx *c, double dt_epoch)
 // voxels of the plume value at the voxel's plume cell, and the block's peak bits;
 // one write per block, no contended atomics
 constexpr int PD_BLOCKS = 2048, PD_THREADS = 256;  // 3 x PD_BLOCKS partials fit the 8192-double scratch
+template <bool W32>
 __global__ __launch_bounds__(PD_THREADS) void k_plume_diag1(const float *__restrict__ E,
                                                             const uint8_t *__restrict__ flags, int Nx, int Ny,
                                                             int Nz, int Px, int Py, int Pz, float ra,
@@ -568,11 +616,9 @@ __global__ __launch_bounds__(PD_THREADS) void k_plume_diag1(const float *__restr
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < NP; c += stride)
         pk = max(pk, __float_as_uint(E[c] * ra));
     auto visit = [&](int64_t c) {
-        const int k = (int)(c % Nz);
-        const int64_t r = c / Nz;
-        const int j = (int)(r % Ny), i = (int)(r / Ny);
-        const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
-        s += (double)(E[pc] * ra);
+        int i, j, k;
+        split_ijk<W32>(c, Ny, Nz, i, j, k);
+        s += (double)(E[plume_of<W32>(i, j, k, Nx, Ny, Nz, Px, Py, Pz)] * ra);
         cnt += 1.0;
     };
     // the removed-voxel mask, 16 flag bytes per load; most chunks hold none.  In
@@ -587,16 +633,13 @@ __global__ __launch_bounds__(PD_THREADS) void k_plume_diag1(const float *__restr
             const uint4 w = f16[q];
             if (!((w.x | w.y | w.z | w.w) & b4)) continue;
             const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-            const int64_t cq = q * 16;
-            int k = (int)(cq % Nz);
-            const int64_t r = cq / Nz;
-            int j = (int)(r % Ny), i = (int)(r / Ny);
+            int i, j, k;
+            split_ijk<W32>(q * 16, Ny, Nz, i, j, k);
             float v[16];
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const bool on = (ws[e >> 2] >> (8 * (e & 3))) & F_ABL;
-                const int64_t pc = ((int64_t)cmap(i, Nx, Px) * Py + cmap(j, Ny, Py)) * Pz + cmap(k, Nz, Pz);
-                v[e] = on ? E[pc] * ra : -1.f;
+                v[e] = on ? E[plume_of<W32>(i, j, k, Nx, Ny, Nz, Px, Py, Pz)] * ra : -1.f;
                 if (++k == Nz) {
                     k = 0;
                     if (++j == Ny) {
@@ -688,8 +731,8 @@ int launch_plume_diag_enqueue(Ctx *c)
     const int I = (int)(((int64_t)(2 * (c->g.nx / 2) + 1) * p.nx) / (2 * (int64_t)c->g.nx));
     const int J = (int)(((int64_t)(2 * (c->g.ny / 2) + 1) * p.ny) / (2 * (int64_t)c->g.ny));
     const float ra = (float)(p.mu_a_max / std::fmax(1e-6, p.mu_max));
-    hipLaunchKernelGGL(k_plume_diag1, dim3(PD_BLOCKS), dim3(PD_THREADS), 0, c->stream, c->Pext, c->flags, c->g.nx,
-                       c->g.ny, c->g.nz, p.nx, p.ny, p.nz, ra, part);
+    hipLaunchKernelGGL(map32_ok(c) ? k_plume_diag1<true> : k_plume_diag1<false>, dim3(PD_BLOCKS), dim3(PD_THREADS), 0,
+                       c->stream, c->Pext, c->flags, c->g.nx, c->g.ny, c->g.nz, p.nx, p.ny, p.nz, ra, part);
     BD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_plume_diag2, dim3(1), dim3(PD_THREADS), 0, c->stream, c->Pext, part, PD_BLOCKS, p.ny, p.nz,
                        ra, I, J, dz, acc);
