@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 #include <string>
+#include <utility>
 
 #include "../../include/bugdef.h"
 #include "../../include/bugdef_fno.h"
@@ -236,6 +238,8 @@ int launch_pack_optics(Ctx *c, bool add_plume);
 int launch_reduce(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t mask_bits,
                   double *out);
 int launch_fill(Ctx *c, void *dst, int kind, int64_t n, double v);
+// zeroes up to four ranges of 8-byte words (doubles or 64-bit counters) in one launch
+int launch_zero_f64(Ctx *c, std::initializer_list<std::pair<void *, int64_t>> ranges);
 int launch_phase_map(Ctx *c, const uint8_t *air_gap_dev, uint8_t *out_dev);
 int launch_slice(Ctx *c, const void *src, int kind, int plane, int index, int stride, float *out, int64_t n);
 int launch_slice_plume(Ctx *c, const float *src, int plane, int index, int stride, float *out, int64_t n);

@@ -900,9 +900,7 @@ static int mc_prepare(Ctx *c, const bd_beam *b, const EpochParams *ep, McArgs &a
         c->lat_n = nd;
     }
     BD_TRY(launch_fill(c, c->AL, K_F32, c->nvox * 2, 0.0));
-    BD_TRY(launch_fill(c, c->lat, K_F64, nd, 0.0));
-    BD_TRY(launch_fill(c, c->red + 16, K_F64, 8, 0.0));
-    BD_TRY(launch_fill(c, c->counter, K_F64, 1, 0.0));
+    BD_TRY(launch_zero_f64(c, {{c->lat, nd}, {c->red + 16, 8}, {c->counter, 1}}));
 
     fill_mc_args(c->g, b, a);
     a.opt = c->optics;

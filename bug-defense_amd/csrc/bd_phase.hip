@@ -292,8 +292,7 @@ int launch_phase_enqueue(Ctx *c, double t_now, const EpochParams *ep)
     a.vol = c->g.dx * c->g.dy * c->g.dz;
     a.ep = ep;
     a.nvox = c->nvox;
-    BD_TRY(launch_fill(c, c->counter + 1, K_F64, 1, 0.0));
-    BD_TRY(launch_fill(c, c->red + 9, K_F64, 1, 0.0));
+    BD_TRY(launch_zero_f64(c, {{c->counter + 1, 1}, {c->red + 9, 1}}));
     // the quad loads of k_phase need 16-byte aligned fields (hipMalloc gives more)
     BD_CHECK(((uintptr_t)c->T | (uintptr_t)c->H | (uintptr_t)c->hlat | (uintptr_t)c->flags |
               (uintptr_t)c->to_remove) % 16 == 0,

@@ -2,6 +2,7 @@
 // FP64 reductions, slices and the phase map.  All are HBM-bound single passes.
 #include "bd_internal.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 namespace bd {
@@ -88,6 +89,19 @@ __global__ void k_fill_f32(float *d, float v, int64_t n)
 __global__ void k_fill_f64(double *d, double v, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = v;
+}
+// Zeroes up to four small 8-byte ranges in one launch (scalar accumulators,
+// counters, per-depth tallies): one dependent launch instead of one per range.
+struct ZeroSet {
+    double *p[4];
+    int64_t n[4];
+};
+__global__ __launch_bounds__(256) void k_zero_f64(ZeroSet z)
+{
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < z.n[r]; i += (int64_t)gridDim.x * blockDim.x)
+            z.p[r][i] = 0.0;
 }
 __global__ void k_fill_u8(uint8_t *d, uint8_t v, int64_t n)
 {
@@ -227,6 +241,24 @@ int reduce_masked_enqueue(Ctx *c, const float *src, int64_t n, const uint8_t *fl
 int reduce_masked(Ctx *c, const float *src, int64_t n, const uint8_t *flags, uint8_t bit, bool invert, double *out)
 {
     return reduce_impl(c, src, K_F32, n, 0, flags, bit, invert ? 1 : 0, out);
+}
+
+int launch_zero_f64(Ctx *c, std::initializer_list<std::pair<void *, int64_t>> ranges)
+{
+    BD_CHECK(ranges.size() <= 4, "launch_zero_f64: at most four ranges");
+    ZeroSet z{};
+    int64_t nmax = 1;
+    int r = 0;
+    for (const auto &rg : ranges) {
+        z.p[r] = (double *)rg.first;
+        z.n[r] = rg.second;
+        nmax = std::max(nmax, rg.second);
+        ++r;
+    }
+    const int nb = blocks_for(nmax, 256) > 64 ? 64 : blocks_for(nmax, 256);
+    hipLaunchKernelGGL(k_zero_f64, dim3(nb), dim3(256), 0, c->stream, z);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
 }
 
 int launch_fill(Ctx *c, void *dst, int kind, int64_t n, double v)
