@@ -70,8 +70,9 @@ static int epoch_impl(bd_ctx *h, const bd_beam *beam, int n_substeps, double t_a
     if (beam->power > 0.0 && !run_mc) {
         BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
     } else if (mc) {
-        BD_TRY(launch_mc_enqueue(c, beam, nullptr, 1, REC_LAG));  // this epoch's record passes run at the next one
-        BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
+        // this epoch's record passes run at the next one; the absorbed power over the intact
+        // cells goes to red[40] with the conversion to Q
+        BD_TRY(launch_mc_enqueue(c, beam, nullptr, 1, REC_LAG, c->red + 40));
     } else {
         BD_TRY(join_records(c));
         lag_drop(c);  // the cleared tally supersedes a lagged launch's
@@ -187,10 +188,10 @@ void plume_diag_from(const double *h, double out[4]);
 
 // after the transport: the absorbed-power total, or the cleared fields of a
 // laser-off epoch
-static int enqueue_after_mc(Ctx *c, const bd_beam *beam)
+static int enqueue_after_mc(Ctx *c, const bd_beam *beam, bool q_summed = false)
 {
     if (beam->power > 0.0) {
-        BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
+        if (!q_summed) BD_TRY(reduce_masked_enqueue(c, c->Q, c->nvox, c->flags, F_ABL, true, c->red + 40));
     } else {
         BD_TRY(join_records(c));
         BD_TRY(launch_fill(c, c->Q, K_F32, c->nvox, 0.0));
@@ -236,8 +237,8 @@ static int enqueue_epoch(Ctx *c, const bd_beam *beam, int n_substeps)
     // path lengths inline by default (records only with BUGDEF_MC_DEFER_FLUENCE=1, their passes
     // then right after the transport on the context stream: with many contexts' captured epochs
     // in flight, a second stream per context cost more than it overlapped)
-    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2, REC_MAIN));
-    BD_TRY(enqueue_after_mc(c, beam));
+    if (beam->power > 0.0) BD_TRY(launch_mc_enqueue(c, beam, c->ep, 2, REC_MAIN, c->red + 40));
+    BD_TRY(enqueue_after_mc(c, beam, true));
     return enqueue_epoch_tail(c, n_substeps);
 }
 

@@ -215,7 +215,12 @@ void al_set(Ctx *c, const float *s, int which);  // a float grid into one half o
 inline void lag_drop(Ctx *c) { c->lag = -1; }  // a new transport or a cleared tally supersedes them
 // enqueue only (no host wait); with ep != null the photon key comes from the device params
 // timed: 0 no events, 1 ev0/ev1 around the transport kernel, 2 graph-external gev[0]/gev[1]
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, int rec_mode = REC_MAIN);
+// q_sum: when set, the conversion of the absorbed weight to Q also sums Q over the intact cells
+// (flags without F_ABL) into *q_sum on the device, in the order of reduce_masked_enqueue
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, int rec_mode = REC_MAIN,
+                      double *q_sum = nullptr);
+// Q = A * scale and the masked sum of Q into *dev_out (one pass over the tally)
+int q_from_a_sum_enqueue(Ctx *c, float scale, const uint8_t *mask, uint8_t bits, bool invert, double *dev_out);
 // One transport launch for n contexts (each with power on, EpochParams in c->ep):
 // prepares each on its own stream, traces all photons in one kernel on a
 // library stream, then queues each context's absorbed-power conversion on its

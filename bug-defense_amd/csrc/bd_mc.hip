@@ -940,9 +940,10 @@ static int64_t mc_blocks(Ctx *c, long long n_photons)
 }
 
 // Absorbed power from the absorbed-weight tally (after the transport).
-static int mc_post(Ctx *c, const bd_beam *b)
+static int mc_post(Ctx *c, const bd_beam *b, double *q_sum = nullptr)
 {
     const float scale = (float)(b->power / ((double)b->n_photons * c->g.dx * c->g.dy * c->g.dz));
+    if (q_sum) return q_from_a_sum_enqueue(c, scale, c->flags, F_ABL, true, q_sum);
     const int nb = blocks_for(c->nvox, 256) > 8192 ? 8192 : blocks_for(c->nvox, 256);
     hipLaunchKernelGGL(k_q_from_a, dim3(nb), dim3(256), 0, c->stream, c->AL, c->Q, scale, c->nvox);
     BD_HIP(hipGetLastError());
@@ -1050,7 +1051,7 @@ int lag_finalize(Ctx *c)
 // REC_MAIN after the kernel on the context stream; REC_LAG (eager epochs) not in this epoch:
 // this launch writes the record buffer the previous lagged launch did not, and that launch's
 // passes, if still pending, run on the side stream beside this transport.
-int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, int rec_mode)
+int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed, int rec_mode, double *q_sum)
 {
     McArgs a;
     BD_TRY(ensure_records(c, b, rec_mode));  // may drop lagged records (when the buffers grow)
@@ -1099,13 +1100,13 @@ int launch_mc_enqueue(Ctx *c, const bd_beam *b, const EpochParams *ep, int timed
             BD_HIP(hipEventRecord(c->ev7, c->stream));
             BD_HIP(hipEventRecord(c->ev6, c->stream));
         }
-        return mc_post(c, b);
+        return mc_post(c, b, q_sum);
     }
     if (timed == 1) BD_HIP(hipEventRecord(c->ev7, c->stream));
     if (defer) BD_TRY(launch_record_passes(c, c->stream, c->rec, c->counter + 3, (float *)c->AL + 1, 2));
     if (timed == 1) BD_HIP(hipEventRecord(c->ev6, c->stream));
     if (timed == 2) BD_HIP(hipEventRecordWithFlags(c->gev[6], c->stream, hipEventRecordExternal));
-    return mc_post(c, b);
+    return mc_post(c, b, q_sum);
 }
 
 // ---- gang transport (k_mc_gang): one launch for several contexts' photons.

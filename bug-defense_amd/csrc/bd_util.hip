@@ -197,6 +197,64 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__re
     if (threadIdx.x == 0) *out = s[0];
 }
 
+// Q = A * scale from the absorbed-weight tally (voxel_mc_3d.py:684), and in the same pass the
+// block partials of the sum of Q over the cells whose flags do not hold `bits` (invert = 1) or
+// do (invert = 0).  The grid, the elements each thread takes, their order and the tree are those
+// of k_reduce_partial<K_F32> (sum) on Q, so the partials are the same bits as a separate
+// reduction of the Q written here, without reading Q back.
+__global__ __launch_bounds__(RED_THREADS) void k_q_sum_partial(const float2 *__restrict__ AL, float *__restrict__ Q,
+                                                                float scale, int64_t n,
+                                                                const uint8_t *__restrict__ mask, uint8_t bits,
+                                                                int invert, double *__restrict__ partial)
+{
+    __shared__ double s[RED_THREADS];
+    double acc = 0.0;
+    int64_t i0 = 0;
+    if (((uintptr_t)Q & 15) == 0 && ((uintptr_t)mask & 3) == 0 && ((uintptr_t)AL & 15) == 0) {
+        const int64_t n4 = n / 4;
+        const float4 *a4 = (const float4 *)AL;
+        float4 *q4 = (float4 *)Q;
+        const uint32_t *m4 = (const uint32_t *)mask;
+        const uint32_t b4 = (uint32_t)bits * 0x01010101u;
+        for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n4; q += (int64_t)gridDim.x * blockDim.x) {
+            const float4 lo = a4[2 * q], hi = a4[2 * q + 1];  // {A, L} of four cells
+            const uint32_t hit = m4[q] & b4;
+            const float vv[4] = {lo.x * scale, lo.z * scale, hi.x * scale, hi.z * scale};
+            q4[q] = make_float4(vv[0], vv[1], vv[2], vv[3]);
+            for (int e = 0; e < 4; ++e) {
+                const bool m = ((hit >> (8 * e)) & 0xffu) != 0;
+                if (m != (invert != 0)) acc = acc + (double)vv[e];
+            }
+        }
+        i0 = n4 * 4;
+    }
+    for (int64_t i = i0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = AL[i].x * scale;
+        Q[i] = v;
+        const bool m = (mask[i] & bits) != 0;
+        if (m != (invert != 0)) acc = acc + (double)v;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+int q_from_a_sum_enqueue(Ctx *c, float scale, const uint8_t *mask, uint8_t bits, bool invert, double *dev_out)
+{
+    BD_TRY(ensure(c, (void **)&c->red, sizeof(double) * (64 + RED_BLOCKS)));
+    double *partial = c->red + 64;
+    hipLaunchKernelGGL(k_q_sum_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->AL, c->Q, scale, c->nvox,
+                       mask, bits, invert ? 1 : 0, partial);
+    BD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->stream, partial, RED_BLOCKS, 0, dev_out);
+    BD_HIP(hipGetLastError());
+    return BD_OK;
+}
+
 static int reduce_enqueue(Ctx *c, const void *src, int kind, int64_t n, int op, const uint8_t *mask, uint8_t bits,
                           int invert, double *dev_out)
 {
